@@ -1,0 +1,148 @@
+/*
+ * kgeo C-ABI: MI355X (gfx950) FP64 analytic Kerr raytracer.
+ *
+ * Plain C types only.  Every array argument is a device pointer (HIP global
+ * memory) owned by the caller; the library never allocates or frees caller
+ * memory.  Kernels run asynchronously on the given HIP stream (NULL = the
+ * default stream); a return of 0 means the work was enqueued.  Negative
+ * returns are errors (see KGEO_E_*), with a message in kgeo_last_error()
+ * (thread-local).  No C++ exception crosses this boundary.
+ *
+ * Layouts: per-pixel arrays are [npix]; per-sample arrays are [ngeo][npix]
+ * (sample-major, pixels contiguous), the layout of the reference's (ngeo,
+ * npix) numpy arrays.  Output pointers marked "optional" may be NULL.
+ *
+ * Numerical problems never fail a call: they are reported per pixel in the
+ * optional status bitmask (KGEO_ST_*), and the affected outputs carry
+ * NaN / inf exactly where the reference's numpy code produces them.
+ */
+#ifndef KGEO_H_
+#define KGEO_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KGEO_ABI_VERSION 1
+
+/* return codes */
+#define KGEO_OK 0
+#define KGEO_E_ARG (-1)       /* invalid argument (size, pointer, range) */
+#define KGEO_E_HIP (-2)       /* HIP runtime / launch error */
+#define KGEO_E_WORKSPACE (-3) /* workspace too small or misaligned */
+
+/* params.flags */
+#define KGEO_DO_PHI_T 1u /* compute t, phi, sigma (the reference always does) */
+
+/* per-pixel status bits */
+#define KGEO_ST_NONFINITE 1u         /* some output sample is NaN or inf */
+#define KGEO_ST_DEGENERATE_ROOTS 2u  /* resolvent root xi0 <= 0 (B = 0) */
+#define KGEO_ST_MINSPIN_VORTICAL 4u  /* |a| < 1e-4 and eta <= 0: the reference raises */
+#define KGEO_ST_LADDER_CAP 8u        /* Jacobi AGM ladder capped */
+#define KGEO_ST_LABEL_TIE 16u        /* rclass 0 (exact tie r4 == r_+) or cube-root tie */
+
+/*
+ * Ray-wide parameters.  a, r_o, th_o, maxtaufrac are the reference's
+ * arguments (kerr_raytracing_ana.py:46-51).  The derived scalars must be
+ * the values numpy gives for the reference's expressions:
+ *   a2 = a**2, sin_th = np.sin(th_o), cos_th = np.cos(th_o),
+ *   cos2_th = np.cos(th_o)**2, rplus = 1 + np.sqrt(1 - a**2),
+ *   rminus = 1 - np.sqrt(1 - a**2)
+ * so that the conserved quantities, and hence the integer labels, come out
+ * bit-identical.  kgeo_fill_params() fills them with the C library's libm.
+ */
+typedef struct kgeo_ray_params {
+  double a, r_o, th_o, maxtaufrac;
+  double a2, sin_th, cos_th, cos2_th, rplus, rminus;
+  int64_t npix, ngeo;
+  uint32_t flags;
+  uint32_t reserved;
+} kgeo_ray_params;
+
+int kgeo_abi_version(void);
+const char* kgeo_last_error(void);
+
+/* Fill the derived scalars of p from p->a and p->th_o (host-side, libm). */
+void kgeo_fill_params(kgeo_ray_params* p);
+
+/* Bytes of device workspace kgeo_raytrace_ana / kgeo_coords_at_tau need for
+ * npix pixels (the per-pixel plan plus 256 bytes of flags). */
+size_t kgeo_workspace_bytes(int64_t npix);
+
+/*
+ * Replaces kerr_raytracing_ana.raytrace_ana (kerr_raytracing_ana.py:46-122):
+ * conserved quantities, radial roots and class, total Mino time, the
+ * linspace(0, maxtaufrac*tau_tot, ngeo) Mino-time grid and the coordinates
+ * at every grid point (coords_at_tau, :126-193).
+ *   alpha, beta      [npix]        screen coordinates
+ *   tau              [ngeo][npix]  Geodesics.mino_times
+ *   t, r, th, ph     [ngeo][npix]  Geodesics.geo_coords; t and ph stay
+ *                                   untouched when KGEO_DO_PHI_T is clear
+ *   sig              [ngeo][npix]  Geodesics.affine_times
+ *   (every output pointer may be NULL to skip that output)
+ *   rclass           [npix] int8   optional: radial_roots rclass (utils.py:366-378)
+ *   thclass          [npix] int8   optional: angular_turning thclass (utils.py:266-272)
+ *   nmax_eq          [npix] int32  optional: Geodesics.nmax_eq (utils.py:63-67)
+ *   tau_tot          [npix]        optional: mino_total (utils.py:382-548)
+ *   status           [npix] uint32 optional: KGEO_ST_* bits
+ *   workspace        >= kgeo_workspace_bytes(npix) bytes, 256-byte aligned
+ */
+int kgeo_raytrace_ana(const kgeo_ray_params* p, const double* alpha, const double* beta, double* tau, double* t,
+                      double* r, double* th, double* ph, double* sig, int8_t* rclass, int8_t* thclass,
+                      int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/*
+ * The two stages of kgeo_raytrace_ana, for callers that re-sample one plan.
+ * kgeo_plan is the per-pixel preliminaries of raytrace_ana
+ * (kerr_raytracing_ana.py:75-94: lambda, eta, radial_roots, mino_total) plus
+ * the tau-independent parts of th_integrate / r_integrate (:217-251,
+ * :300-327, :404-434, :473-510, :554-584), written into `workspace`.
+ * kgeo_sample is coords_at_tau (:97, :126-193) from that plan: tau_in
+ * [ntau][npix] if non-NULL, else the linspace grid of p->ngeo samples
+ * (then ntau must equal p->ngeo).  Outputs as in kgeo_raytrace_ana.
+ */
+int kgeo_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, int8_t* rclass, int8_t* thclass,
+              int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* workspace, size_t workspace_bytes,
+              void* stream);
+int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, double* tau, double* t, double* r,
+                double* th, double* ph, double* sig, uint32_t* status, const void* workspace,
+                size_t workspace_bytes, void* stream);
+
+/*
+ * Replaces kerr_raytracing_ana.coords_at_tau (kerr_raytracing_ana.py:126-193):
+ * coordinates at caller-given Mino times tau_in [ntau][npix].  Outputs as in
+ * kgeo_raytrace_ana with ngeo = ntau (p->ngeo is ignored).
+ */
+int kgeo_coords_at_tau(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau_in,
+                       int64_t ntau, double* t, double* r, double* th, double* ph, double* sig, uint32_t* status,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Replaces n_equatorial_crossings (kerr_raytracing_utils.py:605-684) and
+ * n_poloidal_orbits (kerr_raytracing_utils.py:551-603) at Mino times
+ * tau [ntau][npix]: floor-valued equatorial crossings (-2 for eta <= 0) and
+ * fractional poloidal orbits (-2 for eta <= 0), both [ntau][npix] float64.
+ * Either output may be NULL.  Uses eta without the spin-zero fix, as the
+ * reference does.
+ */
+int kgeo_crossings(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau,
+                   int64_t ntau, double* n_equatorial, double* n_poloidal, void* stream);
+
+/*
+ * Replaces the elliptic plug-in ellippi_arr(n, phi, m)
+ * (scipy_ellip_binding.py:87-237; GSL twin gsl_ellip_binding.py:154-237):
+ * elementwise Pi(n; phi | m) in the A&S convention with the same special
+ * cases (NaN / inf / limits) and the Cauchy principal value for n > 1.
+ * All arrays [count].
+ */
+int kgeo_ellip_pi(const double* n, const double* phi, const double* m, double* out, int64_t count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KGEO_H_ */

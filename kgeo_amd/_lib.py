@@ -1,0 +1,120 @@
+"""ctypes binding of the kgeo C-ABI (include/kgeo.h).
+
+The HIP library ``libkgeo_hip.so`` is built in-tree next to this file by
+``__graft_entry__.build()`` (or ``python -m kgeo_amd.build``).  There is no
+CPU fallback: if the library is missing, every entry point raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkgeo_hip.so")
+
+KGEO_DO_PHI_T = 1
+
+ST_NONFINITE = 1
+ST_DEGENERATE_ROOTS = 2
+ST_MINSPIN_VORTICAL = 4
+ST_LADDER_CAP = 8
+ST_LABEL_TIE = 16
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class RayParams(ctypes.Structure):
+    """kgeo_ray_params (include/kgeo.h)."""
+
+    _fields_ = [
+        ("a", ctypes.c_double), ("r_o", ctypes.c_double), ("th_o", ctypes.c_double),
+        ("maxtaufrac", ctypes.c_double),
+        ("a2", ctypes.c_double), ("sin_th", ctypes.c_double), ("cos_th", ctypes.c_double),
+        ("cos2_th", ctypes.c_double), ("rplus", ctypes.c_double), ("rminus", ctypes.c_double),
+        ("npix", ctypes.c_int64), ("ngeo", ctypes.c_int64),
+        ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+    ]
+
+
+class KgeoError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_PROTOS = {
+    "kgeo_abi_version": (ctypes.c_int, []),
+    "kgeo_last_error": (ctypes.c_char_p, []),
+    "kgeo_fill_params": (None, [ctypes.POINTER(RayParams)]),
+    "kgeo_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
+    "kgeo_raytrace_ana": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 14
+                          + [ctypes.c_size_t, ctypes.c_void_p]),
+    "kgeo_coords_at_tau": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 7
+                           + [ctypes.c_size_t, ctypes.c_void_p]),
+    "kgeo_plan": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 8
+                  + [ctypes.c_size_t, ctypes.c_void_p]),
+    "kgeo_sample": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_int64]
+                    + [ctypes.c_void_p] * 8 + [ctypes.c_size_t, ctypes.c_void_p]),
+    "kgeo_ellip_pi": (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_void_p]),
+    "kgeo_crossings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
+                       + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+
+def exported_symbols():
+    """Names the library must export (every function declared in include/kgeo.h)."""
+    return sorted(_PROTOS)
+
+
+def load():
+    """Load libkgeo_hip.so (raises KgeoError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KgeoError(f"HIP library {LIB_PATH} is missing: run __graft_entry__.build() "
+                        "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.kgeo_abi_version() != 1:
+        raise KgeoError("libkgeo_hip.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().kgeo_last_error().decode(errors="replace")
+        raise KgeoError(f"{what} failed ({rc}): {msg}")
+
+
+def make_params(a, r_o, th_o, npix, ngeo, maxtaufrac, do_phi_t=True):
+    """kgeo_ray_params with the derived scalars computed by numpy exactly as the
+    reference writes them (kerr_raytracing_ana.py:76-77, :390-391), so that the
+    device arithmetic on (alpha, beta) reproduces the reference bit for bit."""
+    p = RayParams()
+    p.a = float(a)
+    p.r_o = float(r_o)
+    p.th_o = float(th_o)
+    p.maxtaufrac = float(maxtaufrac)
+    p.a2 = float(a ** 2)
+    p.sin_th = float(np.sin(th_o))
+    p.cos_th = float(np.cos(th_o))
+    p.cos2_th = float(np.cos(th_o) ** 2)
+    p.rplus = float(1 + np.sqrt(1 - a ** 2))
+    p.rminus = float(1 - np.sqrt(1 - a ** 2))
+    p.npix = int(npix)
+    p.ngeo = int(ngeo)
+    p.flags = KGEO_DO_PHI_T if do_phi_t else 0
+    p.reserved = 0
+    return p
+
+
+def workspace_bytes(npix):
+    return int(load().kgeo_workspace_bytes(int(npix)))

@@ -1,0 +1,139 @@
+// CPU instantiation of the device math (kg_math.hpp / kg_geodesic.hpp).
+//
+// Test and measurement instrument only: it lets the CPU test suite check
+// the exact source the HIP kernels compile (Carlson, Legendre, Jacobi,
+// roots, per-sample evaluation) against the oracle without a GPU, and it
+// backs the algorithmic FLOP counter.  The product modules never load it:
+// kgeo_amd's raytrace entry points require the HIP library and raise if it
+// is missing.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "kg_geodesic.hpp"
+
+using namespace kg;
+
+namespace {
+
+RayScalars<double> scalars_from(const double* s) {
+  // s: a, r_o, th_o, maxtaufrac, a2, sin_th, cos_th, cos2_th, rplus, rminus
+  RayScalars<double> c = make_scalars(s[0], s[2], s[1], s[3]);
+  c.a2 = s[4];
+  c.sin_th = s[5];
+  c.cos_th = s[6];
+  c.cos2_th = s[7];
+  c.rplus = s[8];
+  c.rminus = s[9];
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+void kgh_rf(const double* x, const double* y, const double* z, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = carlson_rf(x[i], y[i], z[i]);
+}
+
+void kgh_rd(const double* x, const double* y, const double* z, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = carlson_rd(x[i], y[i], z[i]);
+}
+
+void kgh_rj(const double* x, const double* y, const double* z, const double* p, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double rf = carlson_rf(x[i], y[i], z[i]);
+    out[i] = carlson_rj_any(x[i], y[i], z[i], p[i], rf);
+  }
+}
+
+void kgh_rc(const double* x, const double* y, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = carlson_rc(x[i], y[i]);
+}
+
+void kgh_ellipj(const double* u, const double* m, int64_t n, double* sn, double* cn, double* dn, double* K) {
+  for (int64_t i = 0; i < n; ++i) {
+    JacobiLadder<double> L;
+    jacobi_ladder(m[i], L);
+    jacobi_eval(L, u[i], sn[i], cn[i], dn[i]);
+    K[i] = L.K;
+  }
+}
+
+// Legendre F, E, Pi at arbitrary amplitude phi (reference conventions,
+// including ellip_pi_arr's special cases).
+void kgh_legendre(const double* nn, const double* phi, const double* m, int64_t n, double* F, double* E, double* P) {
+  for (int64_t i = 0; i < n; ++i) {
+    Amp<double> A = amp_from_angle(phi[i]);
+    LegendreBase<double> L = legendre_base(A, m[i]);
+    double K = ellip_K(m[i]);
+    double EmK = ellip_EmK(m[i]);
+    F[i] = legendre_F(L, K);
+    E[i] = F[i] + legendre_EmF(L, m[i], EmK);
+    double Pc = ellip_pi_complete(nn[i], m[i], K);
+    P[i] = ellip_pi_amp(L, nn[i], m[i], Pc);
+  }
+}
+
+// Setup: labels, tau_tot, roots, u_+-.
+void kgh_setup(const double* sc, const double* alpha, const double* beta, int64_t n, int32_t* rclass, int32_t* thclass,
+               int32_t* rcase, uint32_t* status, double* tau_tot, double* roots8, double* upum) {
+  RayScalars<double> c = scalars_from(sc);
+  for (int64_t i = 0; i < n; ++i) {
+    PixelSetup<double> px;
+    pixel_setup(c, alpha[i], beta[i], px);
+    rclass[i] = px.rclass;
+    thclass[i] = px.thclass;
+    rcase[i] = px.rcase;
+    status[i] = px.status;
+    tau_tot[i] = px.tau_tot;
+    for (int k = 0; k < 4; ++k) {
+      roots8[8 * i + 2 * k] = px.rr[k];
+      roots8[8 * i + 2 * k + 1] = px.ri[k];
+    }
+    upum[2 * i] = px.up;
+    upum[2 * i + 1] = px.um;
+  }
+}
+
+// Full raytrace of the kernels' algorithm on the CPU: outputs [ngeo][npix].
+// tau_in (optional, [ngeo][npix]) replaces the linspace grid.
+void kgh_raytrace(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo, int do_phi_t,
+                  const double* tau_in, double* tau, double* t, double* r, double* th, double* ph, double* sig,
+                  int32_t* nmax_eq) {
+  RayScalars<double> c = scalars_from(sc);
+  std::vector<double> plan(static_cast<size_t>(npix) * PL_N);
+  std::vector<double> stop(npix);
+  bool any_eta_pos = false, any_step_zero = false;
+  for (int64_t i = 0; i < npix; ++i) {
+    PixelSetup<double> px = plan_pixel(c, alpha[i], beta[i], do_phi_t != 0, &plan[i * PL_N]);
+    if (val(px.eta) > 0.0) any_eta_pos = true;
+    double st = plan[i * PL_N + PL_STOP];
+    if (ngeo > 1 && st / static_cast<double>(ngeo - 1) == 0.0) any_step_zero = true;
+    stop[i] = st;
+  }
+  for (int64_t i = 0; i < npix; ++i) {
+    PixelConsts<double> pc;
+    plan_unpack(c, &plan[i * PL_N], pc);
+    bool theta_on = !(pc.th.eta_zero && !any_eta_pos);
+    for (int64_t j = 0; j < ngeo; ++j) {
+      double tj = tau_in ? tau_in[j * npix + i] : mino_tau(j, ngeo, stop[i], any_step_zero);
+      Sample<double> s = eval_sample(pc, tj, do_phi_t != 0, theta_on);
+      int64_t o = j * npix + i;
+      if (tau) tau[o] = tj;
+      if (t) t[o] = s.t;
+      if (r) r[o] = s.r;
+      if (th) th[o] = s.th;
+      if (ph) ph[o] = s.ph;
+      if (sig) sig[o] = s.sig;
+    }
+    if (nmax_eq) {
+      double tl = mino_tau(ngeo - 1, ngeo, stop[i], any_step_zero);
+      nmax_eq[i] = nmax_equatorial_at(c, alpha[i], beta[i], tl);
+    }
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+"""Shared helpers for the test suite: workload definitions, the error
+metrics used for parity, the conditioning probe, and a ctypes view of the
+CPU instantiation of the device headers (libkgeo_hostcheck.so)."""
+
+import ctypes
+import os
+
+import numpy as np
+
+from oracle import kgeo_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOSTCHECK = os.path.join(ROOT, "kgeo_amd", "libkgeo_hostcheck.so")
+
+# Workloads (seeds and unstated choices fixed here)
+C1 = dict(a=0.94, th_deg=17.0, r_o=1.0e4, window=8.0, n=64, ngeo=100)
+C2 = dict(a=0.94, th_deg=17.0, r_o=1.0e4, window=8.0, n=512, ngeo=500, subset=4096, seed=0)
+
+
+def grid(window, n):
+    """alpha, beta = meshgrid(linspace(-w, w, n)) flattened, alpha fastest."""
+    al = np.linspace(-window, window, n)
+    A, B = np.meshgrid(al, al)
+    return A.flatten(), B.flatten()
+
+
+def c2_subset():
+    A, B = grid(C2["window"], C2["n"])
+    idx = np.random.default_rng(C2["seed"]).choice(len(A), C2["subset"], replace=False)
+    return A[idx], B[idx], idx
+
+
+# --- error metrics ---------------------------------------------------------
+# r: pointwise relative.  theta: |d theta| / max(|theta|, 1) (radians for
+# small angles).  phi, t, sigma start at exactly 0 on every ray and
+# accumulate; their pointwise relative error is undefined near tau = 0, so
+# they are measured against the ray's own magnitude: |dx| / max(max_j |x_j|, 1).
+
+def errors(x, y, key):
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    if key == "r":
+        return np.abs(x - y) / np.abs(y)
+    if key == "th":
+        return np.abs(x - y) / np.maximum(np.abs(y), 1.0)
+    scale = np.maximum(np.nanmax(np.abs(y), axis=0, keepdims=True), 1.0)
+    return np.abs(x - y) / scale
+
+
+def conditioning_mask(a, r_o, th_o, alpha, beta, tau, ref, rel=1.0e-14, thresh=1.0e-11):
+    """True where the oracle itself is ill-conditioned at the 1e-10 level.
+
+    * the last sample of horizon-bound rays (rclass 2, 3, 4): tau =
+      (1 - 1e-10) tau_tot leaves r - r_+ known to ~1e-6 relative, and phi, t,
+      sigma contain log(r - r_+), so the reference's own value there moves in
+      steps of ~1e-5 as tau changes by single ulps;
+    * any sample whose oracle value moves by more than `thresh` (same
+      metric) when tau is scaled by (1 -+ rel): this catches the removable
+      log singularities of the reference's R1/S1 split (Pi and f1 both
+      diverge where 1 - n sin^2(am) -> 0) and the near-critical-curve rays.
+    """
+    bad = np.zeros(tau.shape, dtype=bool)
+    horizon = np.isin(ref["rclass"], (2, 3, 4)) if "rclass" in ref else np.zeros(tau.shape[1], bool)
+    bad[-1, horizon] = True
+    for f in (1.0 - rel, 1.0 + rel):
+        sig2, (t2, r2, th2, ph2) = O.coords_at_tau(a, r_o, th_o, alpha, beta, tau * f)
+        pert = dict(t=t2, r=r2, th=th2, ph=ph2, sig=sig2)
+        for k in ("r", "th", "ph", "t", "sig"):
+            bad |= errors(pert[k], ref[k], k) > thresh
+    return bad
+
+
+def summarize(x, y, key, ill=None):
+    e = errors(x, y, key)
+    d = dict(max=float(np.nanmax(e)), p99=float(np.nanpercentile(e, 99)))
+    if ill is not None:
+        ew = np.where(ill, np.nan, e)
+        d["max_well"] = float(np.nanmax(ew))
+        d["max_ill"] = float(np.nanmax(np.where(ill, e, np.nan))) if ill.any() else 0.0
+        d["n_ill"] = int(ill.sum())
+    d["nan_mismatch"] = int(np.sum(np.isnan(np.asarray(x, float)) != np.isnan(np.asarray(y, float))))
+    return d
+
+
+# --- host instantiation of the device headers ------------------------------
+class Host:
+    def __init__(self):
+        if not os.path.exists(HOSTCHECK):
+            raise RuntimeError("libkgeo_hostcheck.so not built: run python -m kgeo_amd.build")
+        self.lib = ctypes.CDLL(HOSTCHECK)
+
+    @staticmethod
+    def _p(a, t=ctypes.c_double):
+        return a.ctypes.data_as(ctypes.POINTER(t))
+
+    def _call(self, name, ins, nout):
+        ins = [np.ascontiguousarray(a, dtype=float) for a in ins]
+        n = len(ins[0])
+        outs = [np.empty(n) for _ in range(nout)]
+        getattr(self.lib, name)(*[self._p(a) for a in ins], ctypes.c_int64(n), *[self._p(o) for o in outs])
+        return outs
+
+    def rf(self, x, y, z):
+        return self._call("kgh_rf", [x, y, z], 1)[0]
+
+    def rd(self, x, y, z):
+        return self._call("kgh_rd", [x, y, z], 1)[0]
+
+    def rj(self, x, y, z, p):
+        return self._call("kgh_rj", [x, y, z, p], 1)[0]
+
+    def rc(self, x, y):
+        return self._call("kgh_rc", [x, y], 1)[0]
+
+    def ellipj(self, u, m):
+        return self._call("kgh_ellipj", [u, m], 4)
+
+    def legendre(self, n, phi, m):
+        return self._call("kgh_legendre", [n, phi, m], 3)
+
+    @staticmethod
+    def scalars(a, r_o, th_o, maxtaufrac=O.MAXTAUFRAC):
+        return np.array([a, r_o, th_o, maxtaufrac, a ** 2, np.sin(th_o), np.cos(th_o), np.cos(th_o) ** 2,
+                         1 + np.sqrt(1 - a ** 2), 1 - np.sqrt(1 - a ** 2)])
+
+    def setup(self, a, r_o, th_o, alpha, beta):
+        n = len(alpha)
+        I = ctypes.c_int32
+        rc = np.empty(n, np.int32); tc = np.empty(n, np.int32); cs = np.empty(n, np.int32)
+        st = np.empty(n, np.uint32); tt = np.empty(n); r8 = np.empty(8 * n); uu = np.empty(2 * n)
+        self.lib.kgh_setup(self._p(self.scalars(a, r_o, th_o)), self._p(np.ascontiguousarray(alpha, float)),
+                           self._p(np.ascontiguousarray(beta, float)), ctypes.c_int64(n), self._p(rc, I),
+                           self._p(tc, I), self._p(cs, I), self._p(st, ctypes.c_uint32), self._p(tt), self._p(r8),
+                           self._p(uu))
+        return dict(rclass=rc, thclass=tc, rcase=cs, status=st, tau_tot=tt, roots=r8.reshape(n, 8),
+                    upum=uu.reshape(n, 2))
+
+    def raytrace(self, a, r_o, th_o, alpha, beta, ngeo, tau_in=None):
+        n = len(alpha)
+        outs = {k: np.empty((ngeo, n)) for k in ("tau", "t", "r", "th", "ph", "sig")}
+        nm = np.empty(n, np.int32)
+        ti = None if tau_in is None else self._p(np.ascontiguousarray(tau_in, float))
+        self.lib.kgh_raytrace(self._p(self.scalars(a, r_o, th_o)), self._p(np.ascontiguousarray(alpha, float)),
+                              self._p(np.ascontiguousarray(beta, float)), ctypes.c_int64(n), ctypes.c_int64(ngeo),
+                              ctypes.c_int(1), ti, *[self._p(outs[k]) for k in ("tau", "t", "r", "th", "ph", "sig")],
+                              self._p(nm, ctypes.c_int32))
+        outs["nmax_eq"] = nm
+        return outs

@@ -1,0 +1,90 @@
+"""The C-ABI library: loads without a GPU, exports every function declared
+in include/kgeo.h, and rejects bad arguments with error codes (host-side
+validation only; nothing here launches a kernel)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from kgeo_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "kgeo.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(kgeo_[a-z_]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == _lib.exported_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "run python -m kgeo_amd.build"
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [s for s in _declared() if s not in exported]
+    assert not missing, missing
+
+
+def test_load_and_version():
+    lib = _lib.load()
+    assert lib.kgeo_abi_version() == 1
+    assert _lib.workspace_bytes(1000) == 256 + 1000 * _plan_slots() * 8
+
+
+def _plan_slots():
+    lib = _lib.load()
+    return (lib.kgeo_workspace_bytes(1) - 256) // 8
+
+
+def test_fill_params_matches_numpy_expressions():
+    import numpy as np
+    lib = _lib.load()
+    p = _lib.RayParams()
+    p.a, p.th_o = 0.94, 17 * np.pi / 180
+    lib.kgeo_fill_params(ctypes.byref(p))
+    q = _lib.make_params(0.94, 1e4, 17 * np.pi / 180, 1, 1, 1.0)
+    for f in ("a2", "sin_th", "cos_th", "cos2_th", "rplus", "rminus"):
+        assert abs(getattr(p, f) - getattr(q, f)) <= 2e-16 * max(1, abs(getattr(q, f))), f
+
+
+@pytest.mark.parametrize("bad", ["null_params", "neg_npix", "no_workspace", "small_workspace", "ngeo0"])
+def test_argument_errors(bad):
+    lib = _lib.load()
+    p = _lib.make_params(0.5, 1e4, 0.5, 10, 10, 1.0)
+    buf = (ctypes.c_char * 8192)()
+    addr = (ctypes.addressof(buf) + 255) // 256 * 256
+    ws, wsb = ctypes.c_void_p(addr), 4096
+    fake = ctypes.c_void_p(addr)   # never dereferenced: validation fails first
+    pp = ctypes.byref(p)
+    if bad == "null_params":
+        pp = None
+    elif bad == "neg_npix":
+        p.npix = -1
+    elif bad == "no_workspace":
+        ws = None
+    elif bad == "small_workspace":
+        wsb = 16
+    elif bad == "ngeo0":
+        p.ngeo = 0
+    rc = lib.kgeo_raytrace_ana(pp, fake, fake, None, None, None, None, None, None, None, None, None, None, None,
+                               ws, wsb, None)
+    assert rc < 0
+    assert lib.kgeo_last_error()
+
+
+def test_product_path_requires_gpu():
+    # no CPU fallback: without a GPU the drop-in raises instead of computing
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from kgeo_amd.raytrace import raytrace_ana
+    with pytest.raises(_lib.KgeoError):
+        raytrace_ana(0.5, [0, 1e4, 0.5, 0], [[1.0], [2.0]], ngeo=4, plotdata=False)

@@ -1,0 +1,188 @@
+"""GPU parity of the HIP raytracer against the oracle, through the C-ABI.
+
+Bar: integer labels (rclass, thclass, nmax_eq, equatorial crossing counts)
+bit-exact; r, theta, phi, t, sigma within 1e-10 (metrics in
+tests/kgeo_testlib.errors) on every sample the oracle itself resolves at
+that level (conditioning_mask), with max and p99 over all samples printed.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import kgeo_oracle as O
+
+from .kgeo_testlib import C1, C2, c2_subset, conditioning_mask, grid, summarize
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _gpu(a, r_o, th, A, B, G, **kw):
+    import torch
+    from kgeo_amd.raytrace import raytrace_ana_device
+    res = raytrace_ana_device(a, r_o, th, A, B, G, **kw)
+    torch.cuda.synchronize()
+    out = {k: getattr(res, k).cpu().numpy() for k in ("tau", "t", "r", "th", "ph", "sig", "rclass", "thclass",
+                                                        "nmax_eq", "tau_tot", "status")}
+    return out
+
+
+def _check(got, ref, a, r_o, th, A, B, label):
+    assert np.array_equal(got["rclass"].astype(int), ref["rclass"]), label
+    assert np.array_equal(got["thclass"].astype(int), ref["thclass"]), label
+    nm = O.n_equatorial_crossings(a, th, A, B, ref["tau"][-1])
+    assert np.array_equal(got["nmax_eq"].astype(float), nm), label
+    assert np.max(np.abs(got["tau"] - ref["tau"]) / np.maximum(ref["tau"], 1e-300)) < 1e-12
+    ill = conditioning_mask(a, r_o, th, A, B, ref["tau"], ref)
+    rep = {}
+    for k in ("r", "th", "ph", "t", "sig"):
+        s = summarize(got[k], ref[k], k, ill)
+        rep[k] = s
+        assert s["nan_mismatch"] == 0, (label, k, s)
+        assert s["max_well"] < 1e-10, (label, k, s)
+        assert s["p99"] < 1e-10, (label, k, s)
+    print(f"\n[{label}] ill-conditioned samples: {int(ill.sum())} of {ill.size}")
+    for k, s in rep.items():
+        print(f"  {k:3s} max {s['max']:.2e}  p99 {s['p99']:.2e}  max(well-conditioned) {s['max_well']:.2e}"
+              f"  max(ill) {s['max_ill']:.2e}")
+    return rep
+
+
+def test_c1_full_image_parity():
+    a, th = C1["a"], np.deg2rad(C1["th_deg"])
+    A, B = grid(C1["window"], C1["n"])
+    ref = O.raytrace_ana(a, C1["r_o"], th, A, B, C1["ngeo"])
+    got = _gpu(a, C1["r_o"], th, A, B, C1["ngeo"])
+    _check(got, ref, a, C1["r_o"], th, A, B, "64x64 ngeo=100")
+
+
+def test_c2_seeded_subset_parity_and_full_image_labels():
+    a, th = C2["a"], np.deg2rad(C2["th_deg"])
+    A, B = grid(C2["window"], C2["n"])
+    got = _gpu(a, C2["r_o"], th, A, B, C2["ngeo"])
+    # labels over all 262144 pixels
+    lam, eta = O.conserved_fixed(a, th, A, B)
+    r1, r2, r3, r4, rclass = O.radial_roots(a, lam, eta)
+    assert np.array_equal(got["rclass"].astype(int), rclass)
+    assert np.array_equal(got["thclass"].astype(int), O.theta_class(eta))
+    tt = O.mino_total(a, C2["r_o"], eta, r1, r2, r3, r4)
+    assert np.max(np.abs(got["tau_tot"] - tt) / tt) < 1e-12
+    last = O.mino_grid(tt, C2["ngeo"])[-1]
+    assert np.array_equal(got["nmax_eq"].astype(float), O.n_equatorial_crossings(a, th, A, B, last))
+    # samples on the seeded 4096-pixel subset
+    As, Bs, idx = c2_subset()
+    ref = O.raytrace_ana(a, C2["r_o"], th, As, Bs, C2["ngeo"])
+    sub = {k: (v[:, idx] if v.ndim == 2 else v[idx]) for k, v in got.items()}
+    rep = _check(sub, ref, a, C2["r_o"], th, As, Bs, "512x512 ngeo=500, 4096 seeded pixels")
+    os.makedirs("gpurun_out", exist_ok=True)
+    json.dump(rep, open(os.path.join("gpurun_out", "parity_c2.json"), "w"), indent=1)
+
+
+def test_full_size_invariants_and_determinism():
+    import torch
+    a, th = C2["a"], np.deg2rad(C2["th_deg"])
+    A, B = grid(C2["window"], C2["n"])
+    os.environ["KGEO_CHUNKS"] = "1"
+    g1 = _gpu(a, C2["r_o"], th, A, B, C2["ngeo"])
+    os.environ["KGEO_CHUNKS"] = "7"
+    g2 = _gpu(a, C2["r_o"], th, A, B, C2["ngeo"])
+    del os.environ["KGEO_CHUNKS"]
+    for k in ("tau", "t", "r", "th", "ph", "sig"):
+        assert np.array_equal(g1[k], g2[k], equal_nan=True), k   # bit-identical across launch shapes
+    rplus = 1 + np.sqrt(1 - a ** 2)
+    assert np.all(g1["status"] & ~0x1 == 0)
+    assert np.all(np.isfinite(g1["r"])) and np.all(np.isfinite(g1["th"]))
+    assert np.all(g1["r"] > rplus * (1 - 1e-12))
+    assert np.all((g1["th"] > 0) & (g1["th"] < np.pi))
+    assert np.all(np.diff(g1["tau"], axis=0) > 0)
+    assert np.max(np.abs(g1["r"][0] / C2["r_o"] - 1)) < 1e-10
+    assert np.max(np.abs(g1["th"][0] - th)) < 1e-10
+    for k in ("ph", "t", "sig"):
+        scale = np.maximum(np.abs(g1[k]).max(axis=0), 1)
+        assert np.max(np.abs(g1[k][0]) / scale) < 1e-12, k
+    del g1, g2
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("case", [
+    (0.0, 30.0, 12.0, 40, 30),        # |a| < MINSPIN branches
+    (-0.9, 80.0, 12.0, 60, 40),       # negative spin, high inclination (case-2 region II rays)
+    (0.7, 140.0, 10.0, 37, 25),       # th_o > pi/2, ragged pixel count
+    (0.99, 5.0, 12.0, 50, 2),         # ngeo = 2
+])
+def test_other_regimes(case):
+    a, th_deg, W, n, G = case
+    th = np.deg2rad(th_deg)
+    A, B = grid(W, n)
+    ref = O.raytrace_ana(a, 1e4, th, A, B, G)
+    got = _gpu(a, 1e4, th, A, B, G)
+    _check(got, ref, a, 1e4, th, A, B, f"a={a} th={th_deg} n={n} G={G}")
+
+
+def test_single_pixel_and_ngeo_one():
+    a, th = 0.5, 0.7
+    ref = O.raytrace_ana(a, 1e3, th, np.array([2.5]), np.array([-1.5]), 1)
+    got = _gpu(a, 1e3, th, np.array([2.5]), np.array([-1.5]), 1)
+    for k in ("tau", "r", "th"):
+        assert np.allclose(got[k], ref[k], rtol=1e-12, atol=1e-12), k
+
+
+def test_drop_in_raytrace_ana_and_ipole_vectors():
+    from kgeo_amd.raytrace import raytrace_ana
+    g = json.load(open(os.path.join(GOLD, "ipole_midplane.json")))
+    for spin, inc, al, be, x0, y0 in g["cases"]:
+        geos = raytrace_ana(a=spin, observer_coords=[0, g["r_o"], inc / 180. * np.pi, 0], ngeo=g["ngeo"],
+                            image_coords=[al, be], plotdata=False)
+        t, r, h, p = (np.array(c)[:, 0] for c in geos.geo_coords)
+        x, y, z = r * np.sin(h) * np.cos(p), r * np.sin(h) * np.sin(p), r * np.cos(h)
+
+        def interp(x0_, xs, zs):
+            if xs[0] > xs[1]:
+                xs, zs = xs[::-1], zs[::-1]
+            return np.interp(x0_, xs, zs)
+        assert np.allclose([interp(0, z, x), interp(0, z, y)], [x0, y0], rtol=g["rtol"]), (spin, inc, al, be)
+    with pytest.raises(Exception, match=r"\|a\| should be a float"):
+        raytrace_ana(a=1, plotdata=False)
+    with pytest.raises(Exception, match="r_o should be a float"):
+        raytrace_ana(observer_coords=[0, 50., 0.3, 0], plotdata=False)
+    with pytest.raises(Exception, match="th_o should be a float"):
+        raytrace_ana(observer_coords=[0, 1e4, np.pi / 2, 0], plotdata=False)
+
+
+def test_coords_at_tau_and_crossings():
+    from kgeo_amd.counts import crossings_device
+    from kgeo_amd.raytrace import coords_at_tau
+    import torch
+    a, th, r_o = 0.27118414089647946, np.deg2rad(60.0), 1e4
+    A, B = grid(12.0, 40)
+    r_s, Ir, Imax, Nmax = O.r_equatorial(a, r_o, th, 0, A, B)
+    ok = Nmax >= 0
+    sig, (t, r, thv, ph) = coords_at_tau(a, [0, r_o, th, 0], [A[ok], B[ok]], Ir[ok])
+    assert np.max(np.abs(1 - thv[0] / (0.5 * np.pi))) < 1e-12
+    assert np.max(np.abs(1 - r[0] / r_s[ok])) < 1e-10
+    tau = np.linspace(0, 1, 50)[:, None] * Imax[None, :]
+    tau[:, ~ok] = 0.5
+    neq, npol = crossings_device(a, th, A, B, tau)
+    torch.cuda.synchronize()
+    assert np.array_equal(neq.cpu().numpy(), O.n_equatorial_crossings(a, th, A, B, tau))
+    want = O.n_poloidal_orbits(a, th, A, B, tau)
+    assert np.allclose(npol.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
+
+
+def test_ellip_pi_plugin():
+    import torch
+    from kgeo_amd import _lib
+    from kgeo_amd.raytrace import _ptr, _stream_handle
+    g = np.load(os.path.join(GOLD, "ellip_pi_mpmath.npz"))
+    n, phi, m = (torch.from_numpy(np.ascontiguousarray(g[k])).cuda() for k in ("n", "phi", "m"))
+    out = torch.empty_like(n)
+    _lib.check(_lib.load().kgeo_ellip_pi(_ptr(n), _ptr(phi), _ptr(m), _ptr(out), n.numel(), _stream_handle()),
+               "kgeo_ellip_pi")
+    torch.cuda.synchronize()
+    P = out.cpu().numpy()
+    assert np.max(np.abs(P - g["Pi"]) / np.maximum(np.abs(g["Pi"]), 1e-3)) < 1e-12
+    want = O.ellip_pi(g["n"], g["phi"], g["m"])
+    assert np.max(np.abs(P - want) / np.maximum(np.abs(want), 1e-3)) < 1e-12
