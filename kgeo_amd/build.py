@@ -21,6 +21,9 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FMA fusion everywhere except code marked `#pragma clang fp contract(off)`
+    # (the per-pixel setup, which must round as numpy does)
+    "-ffp-contract=fast-honor-pragmas",
     "-Wno-unused-command-line-argument",
 ]
 
@@ -46,13 +49,37 @@ def build_hip(extra=()):
     return out
 
 
+def host_clang():
+    for c in (shutil.which("amdclang++"), "/opt/rocm/llvm/bin/clang++", shutil.which("clang++")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
 def build_hostcheck():
+    """CPU build of the device headers with the GPU's floating-point contract:
+    clang, FMA contraction on (as hipcc does for device code), so the CPU
+    suite sees the same rounding decisions (e.g. which setup code must stay
+    uncontracted) as the kernels."""
     out = os.path.join(HERE, "libkgeo_hostcheck.so")
     tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unknown-pragmas", "-ffp-contract=off",
-          "-o", tmp, os.path.join(CSRC, "kg_hostcheck.cpp")])
+    cxx = host_clang()
+    flags = ["-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unknown-pragmas"]
+    if cxx and _cpu_has_fma():
+        flags += ["-mfma", "-mavx2", "-ffp-contract=fast-honor-pragmas"]
+    else:
+        cxx = "g++"
+        flags += ["-ffp-contract=off"]
+    _run([cxx] + flags + ["-o", tmp, os.path.join(CSRC, "kg_hostcheck.cpp")])
     os.replace(tmp, out)
     return out
+
+
+def _cpu_has_fma():
+    try:
+        return " fma " in open("/proc/cpuinfo").read().replace("\n", " ")
+    except OSError:
+        return False
 
 
 def build_opcount():

@@ -82,6 +82,9 @@ template <class T> struct PixelSetup {
 // applies to both; "alias" reproduces that.
 template <class T>
 KG_INLINE void uplus_uminus(const RayScalars<T>& c, const T& lam, const T& eta, T& up, T& um) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   const bool alias = c.minspin;
   if (alias) {
     up = eta / (eta + lam * lam);
@@ -188,6 +191,9 @@ template <class T> KG_INLINE void radial_roots(const RayScalars<T>& c, const T& 
 
 // Amplitude helpers for angles the reference builds with arcsin / arccos.
 template <class T> KG_INLINE Amp<T> amp_from_sin(const T& x) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   Amp<T> A;
   A.j = 0.0;
   A.s = x;
@@ -196,6 +202,9 @@ template <class T> KG_INLINE Amp<T> amp_from_sin(const T& x) {
 }
 
 template <class T> KG_INLINE Amp<T> amp_from_cos(const T& x) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   // phi = arccos(x) in [0, pi]
   Amp<T> A;
   T s = ksqrt((1.0 - x) * (1.0 + x));
@@ -294,6 +303,9 @@ template <class T> struct ThetaConsts {
 
 template <class T>
 KG_INLINE void theta_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bool do_phi_t, ThetaConsts<T>& tc) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   for (int i = 0; i < TH_NSLOT; ++i) tc.v[i] = T(0.0);
   tc.kind = THK_NONE;
   tc.eta_zero = (val(px.eta) == 0.0);
@@ -501,6 +513,9 @@ template <class T> KG_INLINE Amp<T> amp_from_sc(const T& sb, const T& cb) {
 // (the case-4 amplitudes, kerr_raytracing_ana.py:432, utils.py:437-438);
 // x = +inf gives the r_o = infinity limit.
 template <class T> KG_INLINE void atan_sum_sc(const T& x, const T& g, bool x_inf, T& sb, T& cb) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   T dg = ksqrt(1.0 + g * g);
   if (x_inf) {
     sb = 1.0 / dg;
@@ -514,6 +529,9 @@ template <class T> KG_INLINE void atan_sum_sc(const T& x, const T& g, bool x_inf
 
 template <class T>
 KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bool do_phi_t, RadialConsts<T>& rc) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   for (int i = 0; i < R_NSLOT; ++i) rc.v[i] = T(0.0);
   rc.rcase = px.rcase;
   T* v = rc.v;
@@ -800,6 +818,9 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
 // the reference's order so overlapping masks resolve the same way.
 // ---------------------------------------------------------------------------
 template <class T> KG_INLINE T mino_total(const RayScalars<T>& c, const PixelSetup<T>& px) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   const bool iv = !(val(px.ri[0]) == 0.0);
   const bool im3 = !(val(px.ri[2]) == 0.0);
   const T& rplus = c.rplus;
@@ -867,17 +888,18 @@ template <class T> KG_INLINE T mino_total(const RayScalars<T>& c, const PixelSet
 }
 
 // Full pixel setup: conserved quantities (with the spin-zero eta fix),
-// angular turning points, roots, labels and tau_tot.
+// angular turning points, roots, labels and tau_tot.  All per-pixel code
+// is evaluated without FMA contraction so that its rounding is numpy's:
+// u_+ = dth + sqrt(dth^2 + eta/a^2) cancels to ~1e-5 for near-polar rays,
+// and the reference's own rounding there is what the samples inherit.
 template <class T>
 KG_INLINE void pixel_setup(const RayScalars<T>& c, const T& alpha, const T& beta, PixelSetup<T>& px) {
-  px.status = 0;
-  {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
-    px.lam = -alpha * c.sin_th;
-    px.eta = (alpha * alpha - c.a2) * c.cos2_th + beta * beta;
-  }
+  px.status = 0;
+  px.lam = -alpha * c.sin_th;
+  px.eta = (alpha * alpha - c.a2) * c.cos2_th + beta * beta;
   if (c.minspin && val(px.eta) < 0.0) px.eta = T(kEP);
   px.s_o = (val(beta) >= 0.0) ? T(1.0) : (val(beta) < 0.0 ? T(-1.0) : T(0.0));
   uplus_uminus(c, px.lam, px.eta, px.up, px.um);
@@ -1017,6 +1039,9 @@ KG_INLINE PixelSetup<double> plan_pixel(const RayScalars<double>& c, double alph
 // Geodesics.nmax_eq (kerr_raytracing_utils.py:63-67, 605-684) at the last
 // Mino-time sample tau_last; uses eta without the spin-zero fix.
 KG_INLINE int nmax_equatorial_at(const RayScalars<double>& c, double alpha, double beta, double tau_last) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
   double lam = -alpha * c.sin_th;
   double eta = (alpha * alpha - c.a2) * c.cos2_th + beta * beta;
   if (!(eta > 0.0)) return (eta <= 0.0) ? -2 : 0;

@@ -47,26 +47,48 @@ def errors(x, y, key):
     return np.abs(x - y) / scale
 
 
-def conditioning_mask(a, r_o, th_o, alpha, beta, tau, ref, rel=1.0e-14, thresh=1.0e-11):
-    """True where the oracle itself is ill-conditioned at the 1e-10 level.
+def conditioning_mask(a, r_o, th_o, alpha, beta, tau, ref, rel=1.0e-14, ulps=2, thresh=1.0e-11):
+    """True where the oracle itself is ill-conditioned at the 1e-10 level,
+    i.e. where the reference's own rounding noise can reach ~1e-10:
 
     * the last sample of horizon-bound rays (rclass 2, 3, 4): tau =
       (1 - 1e-10) tau_tot leaves r - r_+ known to ~1e-6 relative, and phi, t,
-      sigma contain log(r - r_+), so the reference's own value there moves in
-      steps of ~1e-5 as tau changes by single ulps;
+      sigma contain log(r - r_+);
     * any sample whose oracle value moves by more than `thresh` (same
-      metric) when tau is scaled by (1 -+ rel): this catches the removable
-      log singularities of the reference's R1/S1 split (Pi and f1 both
-      diverge where 1 - n sin^2(am) -> 0) and the near-critical-curve rays.
+      metric) when tau is scaled by (1 -+ rel), or when the intermediates
+      the samples inherit are moved by -+`ulps` ulp: u_+ and u_- (for
+      near-polar rays u_+ = dth + sqrt(dth^2 + eta/a^2) cancels to ~1e-7 and
+      phi varies as 1/sin^2(theta) near the pole) and the radial roots (near
+      the critical curve r3 ~ r4).  This also catches the removable log
+      singularities of the reference's R1/S1 split.
     """
     bad = np.zeros(tau.shape, dtype=bool)
     horizon = np.isin(ref["rclass"], (2, 3, 4)) if "rclass" in ref else np.zeros(tau.shape[1], bool)
     bad[-1, horizon] = True
+
+    def flag(pert):
+        nonlocal bad
+        for k in pert:
+            bad |= errors(pert[k], ref[k], k) > thresh
+
     for f in (1.0 - rel, 1.0 + rel):
         sig2, (t2, r2, th2, ph2) = O.coords_at_tau(a, r_o, th_o, alpha, beta, tau * f)
-        pert = dict(t=t2, r=r2, th=th2, ph=ph2, sig=sig2)
-        for k in ("r", "th", "ph", "t", "sig"):
-            bad |= errors(pert[k], ref[k], k) > thresh
+        flag(dict(t=t2, r=r2, th=th2, ph=ph2, sig=sig2))
+
+    lam, eta = O.conserved_fixed(a, th_o, alpha, beta)
+    s_o = O.sign_ge0(beta)
+    up, um, _, _ = O.uplus_uminus(a, th_o, lam, eta)
+    r1, r2, r3, r4, _ = O.radial_roots(a, lam, eta)
+    rs, Iph, It, Isig = O.r_integrate(a, r_o, lam, eta, r1, r2, r3, r4, tau)
+    thb, Gph, Gt = O.th_integrate(a, th_o, s_o, lam, eta, up, um, tau)
+    eps = ulps * 2.0 ** -52
+    with np.errstate(all="ignore"):
+        for f in (1.0 - eps, 1.0 + eps):
+            for upp, ump in ((up * f, um), (up, um * f)):
+                th2, Gph2, Gt2 = O.th_integrate(a, th_o, s_o, lam, eta, upp, ump, tau)
+                flag(dict(th=th2, ph=Iph + lam * Gph2, t=It + a ** 2 * Gt2, sig=Isig + a ** 2 * Gt2))
+            rs2, Iph2, It2, Isig2 = O.r_integrate(a, r_o, lam, eta, r1 * f, r2 * f, r3 * f, r4 * f, tau)
+            flag(dict(r=rs2, ph=Iph2 + lam * Gph, t=It2 + a ** 2 * Gt, sig=Isig2 + a ** 2 * Gt))
     return bad
 
 
