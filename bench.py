@@ -208,7 +208,7 @@ def main():
     achieved_tf = flops_s * samples / (samp_ms * 1e-3) / 1e12
     bytes_per_launch = 32.0 * samples
     hbm_gbs = bytes_per_launch / (samp_ms * 1e-3) / 1e9
-    roofline = {"bound": "valu-fp64", "kernel": "kg_samples", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+    roofline = {"bound": "valu-fp64", "kernel": "kg_theta + kg_radial", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
                 "flops_per_sample": flops_s, "algorithmic_bytes_per_launch": bytes_per_launch,
                 "kernel_ms": samp_ms, "hbm_write_GBs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS,

@@ -141,6 +141,38 @@ int kgeo_crossings(const kgeo_ray_params* p, const double* alpha, const double* 
  */
 int kgeo_ellip_pi(const double* n, const double* phi, const double* m, double* out, int64_t count, void* stream);
 
+/*
+ * Equatorial-emission image (workload composed of equatorial_lensing.py
+ * r_equatorial :85-196 and equatorial_images.py :175, :266-334 with the
+ * velocities.py u_kep :229-281 disk): for subrings mbar = 0..mbar_max (<= 3),
+ * I_mbar = g^3 r_s^-emis_p where the subring emits (not N_max < mbar,
+ * N_max == -1 or r_s <= r_+), image = sum_mbar I_mbar.
+ *   image [npix]; I_sub, g_sub, r_s_sub, Ir_sub [mbar_max+1][npix] (r_s and
+ *   I_r as r_equatorial returns them); Imax [npix]; Nmax [npix] int32 (-2 for
+ *   eta <= 0); zero_mask [npix] uint8, bit mbar set where subring mbar is
+ *   masked.  Every output may be NULL.  Uses p->a, th_o, r_o and the
+ *   derived scalars; p->ngeo and p->maxtaufrac are ignored.
+ */
+int kgeo_equatorial_image(const kgeo_ray_params* p, const double* alpha, const double* beta, int mbar_max,
+                          double emis_p, double* image, double* I_sub, double* g_sub, double* r_s_sub,
+                          double* Ir_sub, double* Imax, int32_t* Nmax, uint8_t* zero_mask, void* stream);
+
+/*
+ * No counterpart in the reference (it has no gradients).  Vector-Jacobian
+ * product of kgeo_equatorial_image with respect to (a, th_o, emis_p):
+ * grad3[k] = sum_i grad_image[i] * d image[i] / d theta_k, each pixel's
+ * tangent computed by recomputing its geodesic with forward-mode dual
+ * numbers (no sample storage), reduced deterministically (fixed order, no
+ * atomics).  grad_image may be NULL (all ones).  jac [3][npix] (optional)
+ * receives the per-pixel tangents.  workspace >=
+ * kgeo_equatorial_vjp_workspace_bytes(npix) device bytes.
+ */
+size_t kgeo_equatorial_vjp_workspace_bytes(int64_t npix);
+int kgeo_equatorial_image_vjp(const kgeo_ray_params* p, const double* alpha, const double* beta, int mbar_max,
+                              double emis_p, const double* grad_image, double* jac, double* grad3, void* workspace,
+                              size_t workspace_bytes, void* stream);
+const char* kgeo_equatorial_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

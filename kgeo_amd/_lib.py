@@ -59,6 +59,13 @@ _PROTOS = {
     "kgeo_sample": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_int64]
                     + [ctypes.c_void_p] * 8 + [ctypes.c_size_t, ctypes.c_void_p]),
     "kgeo_ellip_pi": (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_void_p]),
+    "kgeo_equatorial_image": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_double] + [ctypes.c_void_p] * 9),
+    "kgeo_equatorial_vjp_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
+    "kgeo_equatorial_image_vjp": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 4
+                                  + [ctypes.c_size_t, ctypes.c_void_p]),
+    "kgeo_equatorial_last_error": (ctypes.c_char_p, []),
     "kgeo_crossings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
                        + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
@@ -90,7 +97,9 @@ def load():
 
 def check(rc, what):
     if rc != 0:
-        msg = load().kgeo_last_error().decode(errors="replace")
+        lib = load()
+        err = lib.kgeo_equatorial_last_error if what.startswith("kgeo_equatorial") else lib.kgeo_last_error
+        msg = err().decode(errors="replace")
         raise KgeoError(f"{what} failed ({rc}): {msg}")
 
 

@@ -241,6 +241,39 @@ template <class T> KG_INLINE T ellip_pi_amp(const LegendreBase<T>& L, const T& n
   return legendre_Pi(L, n, Pic);
 }
 
+// F, E - F and NP third-kind integrals at one amplitude from one fused
+// Carlson family (the per-sample path).  Special cases of ellip_pi_arr are
+// applied per member exactly as ellip_pi_amp does.
+template <class T, int NP> struct LegendreFam {
+  T F, EmF;
+  T Pi[NP];
+};
+
+template <class T, int NP>
+KG_INLINE void legendre_family(const Amp<T>& A, const T& m, const T& Kc, const T& EmKc, const T (&n)[NP],
+                               const T (&Pic)[NP], bool want_rd, LegendreFam<T, NP>& out) {
+  LegendreBase<T> L;
+  L.s = A.s;
+  L.j = A.j;
+  L.x = A.c * A.c;
+  const T s2 = A.s * A.s;
+  L.y = 1.0 - m * s2;
+  T rho[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) rho[i] = 1.0 - n[i] * s2;
+  CarlsonSet<T, NP> cs;
+  carlson_family(L.x, L.y, rho, want_rd, cs);
+  L.rf = cs.rf;
+  out.F = periods(L.j, Kc) + L.s * cs.rf;
+  out.EmF = want_rd ? periods(L.j, EmKc) - (m * (1.0 / 3.0)) * (s2 * L.s) * cs.rd : T(0.0);
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    bool hit;
+    T sv = pi_special(L, n[i], m, hit);
+    out.Pi[i] = hit ? sv : periods(L.j, Pic[i]) + L.s * cs.rf + (n[i] * (1.0 / 3.0)) * (s2 * L.s) * cs.rj[i];
+  }
+}
+
 // complete Pi(n|m) with the reference's special cases at phi = pi/2
 template <class T> KG_INLINE T ellip_pi_complete(const T& n, const T& m, const T& Kc) {
   if (val(m) < -1.0e14) return T(0.0);
@@ -398,31 +431,35 @@ KG_INLINE void theta_consts(const RayScalars<T>& c, const PixelSetup<T>& px, boo
   }
 }
 
-template <class T>
+// KIND = THK_* fixes the motion class at compile time (the kernels' loops);
+// KIND = -1 reads it from tc.kind.
+template <class T, int KIND = -1>
 KG_INLINE void theta_eval(const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, const T& tau, bool do_phi_t, T& th,
                           T& Gph, T& Gt) {
   const T* v = tc.v;
+  const int kind = (KIND >= 0) ? KIND : tc.kind;
   T snarg = v[TH_w] * (-tau + v[TH_sG]);
   NegAmp<T> R = neg_amplitude(snarg, v[TH_sq], lad);
-  if (tc.kind == THK_VORT) {
+  if (kind == THK_VORT) {
     th = kacos(v[TH_h] * ksqrt(v[TH_A] + v[TH_B] * (R.sinphi * R.sinphi)));
   } else {
     th = kacos(v[TH_A] * R.sinphi);
   }
   if (!do_phi_t) return;
   const T& k = v[TH_k];
-  LegendreBase<T> L = legendre_base(R.A, k);
-  Gph = v[TH_pref] * ellip_pi_amp(L, v[TH_n], k, v[TH_Pic]) - v[TH_sGph];
-  if (tc.kind == THK_ORD) {
-    T EmF = legendre_EmF(L, k, v[TH_EmKk]);
-    Gt = -(v[TH_B] * EmF) / (2.0 * k) - v[TH_sGt];
-  } else if (tc.kind == THK_ORD_K0) {
+  LegendreFam<T, 1> fam;
+  const T nn[1] = {v[TH_n]};
+  const T pc[1] = {v[TH_Pic]};
+  legendre_family(R.A, k, v[TH_Kk], v[TH_EmKk], nn, pc, kind != THK_ORD_K0, fam);
+  Gph = v[TH_pref] * fam.Pi[0] - v[TH_sGph];
+  if (kind == THK_ORD) {
+    Gt = -(v[TH_B] * fam.EmF) / (2.0 * k) - v[TH_sGt];
+  } else if (kind == THK_ORD_K0) {
     // k == 0: m' = 0, am(u|0) = u (kerr_raytracing_ana.py:290-291)
     T Phi = snarg;
     Gt = -v[TH_B] * 0.125 * (-2.0 * Phi + ksin(2.0 * Phi)) - v[TH_sGt];
   } else {
-    T E = legendre_F(L, v[TH_Kk]) + legendre_EmF(L, k, v[TH_EmKk]);
-    Gt = v[TH_C] * E - v[TH_sGt];
+    Gt = v[TH_C] * (fam.F + fam.EmF) - v[TH_sGt];
   }
 }
 
@@ -456,12 +493,11 @@ template <class T> struct RadialConsts {
 // R1/R2 building blocks (kerr_raytracing_ana.py:666-683), given sin, cos of
 // the (unreduced) amplitude, sqrt(1 - j sin^2), and the shared Legendre base.
 template <class T>
-KG_INLINE T R1_block(const T& al, const T& nn, const T& Pic, const LegendreBase<T>& L, const T& sphi,
-                     const T& s2phi, const T& j) {
+KG_INLINE T R1_block(const T& al, const T& Pi, const T& sphi, const T& s2phi, const T& j) {
   T al2 = al * al;
   T p1 = ksqrt((al2 - 1.0) / (j + (1.0 - j) * al2));
   T f1 = 0.5 * p1 * klog(kfabs((p1 * s2phi + sphi) / (p1 * s2phi - sphi)));
-  return (ellip_pi_amp(L, nn, j, Pic) - al * f1) / (1.0 - al2);
+  return (Pi - al * f1) / (1.0 - al2);
 }
 
 template <class T>
@@ -476,13 +512,13 @@ KG_INLINE T R2_block(const T& al, const T& nn, const T& R1, const T& F, const T&
 // S1/S2 building blocks (kerr_raytracing_ana.py:638-663).  tan(phi) enters
 // only through (al - tan)/(1 + al tan) = (al c - s)/(c + al s).
 template <class T>
-KG_INLINE T S1_block(const T& al, const T& Pic, const LegendreBase<T>& L, const T& F, const T& s2phi, const T& j) {
+KG_INLINE T S1_block(const T& al, const T& Pi, const T& F, const T& s2phi, const T& j) {
   T al2 = al * al;
   T al2p1 = 1.0 + al2;
   T al2j = 1.0 - j + al2;
   T p2 = ksqrt(al2p1 / al2j);
   T f2 = 0.5 * p2 * klog(kfabs(((1.0 - p2) * (1.0 + p2 * s2phi)) / ((1.0 + p2) * (1.0 - p2 * s2phi))));
-  return (F + al2 * ellip_pi_amp(L, al2p1, j, Pic) - al * f2) / al2p1;
+  return (F + al2 * Pi - al * f2) / al2p1;
 }
 
 template <class T>
@@ -575,11 +611,11 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C4_n0] = n0; v[C4_np] = np; v[C4_nm] = nm;
       v[C4_Pc0] = Pc0; v[C4_Pcp] = Pcp; v[C4_Pcm] = Pcm;
       T Eb = Fb + legendre_EmF(Lb, k4, EmK);
-      T S1b0 = S1_block(g0, Pc0, Lb, Fb, s2b, k4);
+      T S1b0 = S1_block(g0, ellip_pi_amp(Lb, n0, k4, Pc0), Fb, s2b, k4);
       v[C4_S1b0] = S1b0;
       v[C4_S2b0] = S2_block(g0, S1b0, Fb, Eb, sb, cb, s2b, k4);
-      v[C4_S1bp] = S1_block(gp, Pcp, Lb, Fb, s2b, k4);
-      v[C4_S1bm] = S1_block(gm, Pcm, Lb, Fb, s2b, k4);
+      v[C4_S1bp] = S1_block(gp, ellip_pi_amp(Lb, np, k4, Pcp), Fb, s2b, k4);
+      v[C4_S1bm] = S1_block(gm, ellip_pi_amp(Lb, nm, k4, Pcm), Fb, s2b, k4);
       T aa0 = (a2 / g0) * (1.0 + g0 * g0);
       v[C4_P1f] = pref * aa0;
       v[C4_P2f] = pref * (aa0 * aa0);
@@ -641,11 +677,11 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C3_nn0] = nn0; v[C3_nnp] = nnp; v[C3_nnm] = nnm;
       v[C3_Pc0] = Pc0; v[C3_Pcp] = Pcp; v[C3_Pcm] = Pcm;
       T Eb = Fb + legendre_EmF(Lb, k3, EmK);
-      T R1b0 = R1_block(al0, nn0, Pc0, Lb, sb, s2b, k3);
+      T R1b0 = R1_block(al0, ellip_pi_amp(Lb, nn0, k3, Pc0), sb, s2b, k3);
       v[C3_R1b0] = R1b0;
       v[C3_R2b0] = R2_block(al0, nn0, R1b0, Fb, Eb, sb, cb, s2b, k3);
-      v[C3_R1bp] = R1_block(alp, nnp, Pcp, Lb, sb, s2b, k3);
-      v[C3_R1bm] = c.spin_gt_min ? R1_block(alm, nnm, Pcm, Lb, sb, s2b, k3) : T(0.0);
+      v[C3_R1bp] = R1_block(alp, ellip_pi_amp(Lb, nnp, k3, Pcp), sb, s2b, k3);
+      v[C3_R1bm] = c.spin_gt_min ? R1_block(alm, ellip_pi_amp(Lb, nnm, k3, Pcm), sb, s2b, k3) : T(0.0);
       v[C3_hasm] = c.spin_gt_min ? T(1.0) : T(0.0);
       T sqAB = ksqrt(AA * BB);
       T c0 = (2.0 * rr21 * sqAB) / (BB * BB - AA * AA);
@@ -712,11 +748,13 @@ template <class T> struct RadialOut {
 };
 
 // r(tau) and, with do_phi_t, the building blocks I_+, I_-, I_1, I_2
-template <class T>
+// CASE = 2, 3, 4 fixes the radial case at compile time; CASE = -1 reads rc.rcase.
+template <class T, int CASE = -1>
 KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder<T>& lad, const T& tau,
                                    bool do_phi_t) {
   RadialOut<T> o;
   const T* v = rc.v;
+  const int rcase = (CASE >= 0) ? CASE : rc.rcase;
   const T& k = v[R_k];
   T sn, cn, dn;
   T dv = v[R_Xs] * tau;
@@ -735,7 +773,7 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
   } else {
     jacobi_eval(lad, X, sn, cn, dn);
   }
-  if (rc.rcase == 4) {
+  if (rcase == 4) {
     const T& a2 = v[C4_a2];
     const T& b1 = v[C4_b1];
     const T& g0 = v[C4_g0];
@@ -743,13 +781,16 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     o.r = -a2 * (g0 * cn - sn) / (cn + g0 * sn) - b1;
     if (!do_phi_t) return o;
     Amp<T> A = amp_from_jacobi(sn, cn, val(X), val(v[R_K]));
-    LegendreBase<T> L = legendre_base(A, k);
-    T F = legendre_F(L, v[R_K]);
-    T E = F + legendre_EmF(L, k, v[R_EmK]);
-    T S1a0 = S1_block(g0, v[C4_Pc0], L, F, dn, k);
+    LegendreFam<T, 3> fam;
+    const T nn[3] = {v[C4_n0], v[C4_np], v[C4_nm]};
+    const T pc[3] = {v[C4_Pc0], v[C4_Pcp], v[C4_Pcm]};
+    legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
+    const T& F = fam.F;
+    T E = F + fam.EmF;
+    T S1a0 = S1_block(g0, fam.Pi[0], F, dn, k);
     T S2a0 = S2_block(g0, S1a0, F, E, sn, cn, dn, k);
-    T S1ap = S1_block(v[C4_gp], v[C4_Pcp], L, F, dn, k);
-    T S1am = S1_block(v[C4_gm], v[C4_Pcm], L, F, dn, k);
+    T S1ap = S1_block(v[C4_gp], fam.Pi[1], F, dn, k);
+    T S1am = S1_block(v[C4_gm], fam.Pi[2], F, dn, k);
     T Pi1 = v[C4_P1f] * (S1a0 - v[C4_S1b0]);
     T Pi2 = v[C4_P2f] * (S2a0 - v[C4_S2b0]);
     T Pip = v[C4_Ppf] * (S1ap - v[C4_S1bp]);
@@ -759,7 +800,7 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     o.I2 = (aa1 * aa1) * (-tau) - 2.0 * aa1 * Pi1 + Pi2;
     o.Ip = v[C4_Ipf] * (-tau - Pip);
     o.Im = v[C4_Imf] * (-tau - Pim);
-  } else if (rc.rcase == 3) {
+  } else if (rcase == 3) {
     const T& AA = v[C3_AA];
     const T& BB = v[C3_BB];
     const T& rr1 = v[C3_rr1];
@@ -769,13 +810,17 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     o.r = num / den;
     if (!do_phi_t) return o;
     Amp<T> A = amp_from_jacobi(sn, cn, val(X), val(v[R_K]));
-    LegendreBase<T> L = legendre_base(A, k);
-    T F = legendre_F(L, v[R_K]);
-    T E = F + legendre_EmF(L, k, v[R_EmK]);
-    T R1a0 = R1_block(v[C3_al0], v[C3_nn0], v[C3_Pc0], L, sn, dn, k);
+    const bool hasm = val(v[C3_hasm]) != 0.0;
+    LegendreFam<T, 3> fam;
+    const T nn[3] = {v[C3_nn0], v[C3_nnp], hasm ? v[C3_nnm] : T(0.0)};
+    const T pc[3] = {v[C3_Pc0], v[C3_Pcp], v[C3_Pcm]};
+    legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
+    const T& F = fam.F;
+    T E = F + fam.EmF;
+    T R1a0 = R1_block(v[C3_al0], fam.Pi[0], sn, dn, k);
     T R2a0 = R2_block(v[C3_al0], v[C3_nn0], R1a0, F, E, sn, cn, dn, k);
-    T R1ap = R1_block(v[C3_alp], v[C3_nnp], v[C3_Pcp], L, sn, dn, k);
-    T R1am = (val(v[C3_hasm]) != 0.0) ? R1_block(v[C3_alm], v[C3_nnm], v[C3_Pcm], L, sn, dn, k) : T(0.0);
+    T R1ap = R1_block(v[C3_alp], fam.Pi[1], sn, dn, k);
+    T R1am = hasm ? R1_block(v[C3_alm], fam.Pi[2], sn, dn, k) : T(0.0);
     T Pi1 = v[C3_c0] * (R1a0 - v[C3_R1b0]);
     T Pi2 = (v[C3_c0] * v[C3_c0]) * (R2a0 - v[C3_R2b0]);
     T Pip = v[C3_cp] * (R1ap - v[C3_R1bp]);
@@ -799,12 +844,15 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     T drs = -(rr31 * rr43 * rr41 * dsn2 / (den * den));
     T H = drs / (o.r - rr3) - v[C2_drso] / v[C2_ro3];
     Amp<T> A = amp_from_jacobi(sn, cn, val(X), val(v[R_K]));
-    LegendreBase<T> L = legendre_base(A, k);
-    T Ea = legendre_F(L, v[R_K]) + legendre_EmF(L, k, v[R_EmK]);
+    LegendreFam<T, 3> fam;
+    const T nn[3] = {v[C2_n1], v[C2_np], v[C2_nm]};
+    const T pc[3] = {v[C2_Pc1], v[C2_Pcp], v[C2_Pcm]};
+    legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
+    T Ea = fam.F + fam.EmF;
     T E = v[C2_sq] * (Ea - v[C2_Eb]);
-    T Pi1 = v[C2_c] * (ellip_pi_amp(L, v[C2_n1], k, v[C2_Pc1]) - v[C2_Pb1]);
-    T Pip = v[C2_fp] * (ellip_pi_amp(L, v[C2_np], k, v[C2_Pcp]) - v[C2_Pbp]);
-    T Pim = v[C2_fm] * (ellip_pi_amp(L, v[C2_nm], k, v[C2_Pcm]) - v[C2_Pbm]);
+    T Pi1 = v[C2_c] * (fam.Pi[0] - v[C2_Pb1]);
+    T Pip = v[C2_fp] * (fam.Pi[1] - v[C2_Pbp]);
+    T Pim = v[C2_fm] * (fam.Pi[2] - v[C2_Pbm]);
     o.I1 = rr3 * (-tau) + rr43 * Pi1;
     o.I2 = H - v[C2_h12] * (-tau) - E;
     o.Ip = tau / v[C2_rrp3] - Pip;

@@ -12,6 +12,8 @@
 #include <cstring>
 #include <vector>
 
+#include "kg_dual.hpp"
+#include "kg_equatorial.hpp"
 #include "kg_geodesic.hpp"
 
 using namespace kg;
@@ -133,6 +135,44 @@ void kgh_raytrace(const double* sc, const double* alpha, const double* beta, int
       double tl = mino_tau(ngeo - 1, ngeo, stop[i], any_step_zero);
       nmax_eq[i] = nmax_equatorial_at(c, alpha[i], beta[i], tl);
     }
+  }
+}
+
+// Equatorial image (kg_equatorial.hpp) on the CPU: per-subring arrays are
+// [mbar_max+1][n].
+void kgh_equatorial(const double* sc, const double* alpha, const double* beta, int64_t n, int mbar_max, double p,
+                    double* image, double* I, double* g, double* rs, double* Ir, double* Imax, int32_t* Nmax,
+                    int32_t* zmask) {
+  RayScalars<double> c = scalars_from(sc);
+  const double ri = r_isco(c.a);
+  for (int64_t i = 0; i < n; ++i) {
+    EqPixel<double> e;
+    equatorial_pixel(c, alpha[i], beta[i], p, ri, mbar_max, e);
+    image[i] = e.image;
+    for (int m = 0; m <= mbar_max; ++m) {
+      I[m * n + i] = e.I[m];
+      g[m * n + i] = e.g[m];
+      rs[m * n + i] = e.r_s[m];
+      Ir[m * n + i] = e.Ir[m];
+    }
+    Imax[i] = e.Imax;
+    Nmax[i] = e.Nmax;
+    zmask[i] = e.zero_mask;
+  }
+}
+
+// dImage/d(a, th_o, p) per pixel with forward-mode duals: jac [3][n].
+void kgh_equatorial_jac(double a, double th_o, double r_o, double p, const double* alpha, const double* beta,
+                        int64_t n, int mbar_max, double* image, double* jac) {
+  using D = kgd::Dual<3>;
+  const D ad = D::seed(a, 0), thd = D::seed(th_o, 1), pd = D::seed(p, 2);
+  RayScalars<D> c = make_scalars(ad, thd, r_o, 1.0);
+  const D ri = r_isco(ad);
+  for (int64_t i = 0; i < n; ++i) {
+    EqPixel<D> e;
+    equatorial_pixel(c, D(alpha[i]), D(beta[i]), pd, ri, mbar_max, e);
+    image[i] = e.image.v;
+    for (int k = 0; k < 3; ++k) jac[k * n + i] = e.image.d[k];
   }
 }
 

@@ -294,6 +294,133 @@ KG_INLINE T legendre_Pi(const LegendreBase<T>& L, const T& n, const T& Pic) {
   return periods(L.j, Pic) + L.s * L.rf + (n * (1.0 / 3.0)) * (s2 * L.s) * rj;
 }
 
+// ---------------------------------------------------------------------------
+// Fused Carlson family at z = 1: R_F(x,y,1), R_D(x,y,1) and R_J(x,y,1,rho_i)
+// for NP values rho_i (any sign; rho <= 0 via the reference's DLMF 19.20.14
+// transform), sharing one duplication sequence of (x, y, z).  Per
+// duplication step this costs 3 + NP square roots instead of 3 + 3 + 4 NP
+// for separate evaluations.  The step count is the largest any member needs.
+// ---------------------------------------------------------------------------
+template <class T, int NP> struct CarlsonSet {
+  T rf, rd;
+  T rj[NP];
+};
+
+template <class T, int NP>
+KG_INLINE void carlson_family(const T& x0, const T& y0, const T (&rho)[NP], bool want_rd, CarlsonSet<T, NP>& out) {
+  const T one = T(1.0);
+  T p0[NP], q[NP];
+  bool neg[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    neg[i] = !(val(rho[i]) > 0.0);
+    if (neg[i]) {
+      q[i] = -rho[i];
+      p0[i] = (x0 + y0 + q[i] - x0 * y0) / (one + q[i]);
+    } else {
+      q[i] = T(0.0);
+      p0[i] = rho[i];
+    }
+  }
+  // means and initial deviations of each member
+  const T AF0 = (x0 + y0 + one) * (1.0 / 3.0);
+  const T AD0 = (x0 + y0 + 3.0) * 0.2;
+  T AJ0[NP], delta[NP];
+  double Q = kmax(kmax(kfabs(val(AF0 - x0)), kfabs(val(AF0 - y0))), kfabs(val(AF0 - one))) / kfabs(val(AF0));
+  if (want_rd) Q = kmax(Q, kmax(kmax(kfabs(val(AD0 - x0)), kfabs(val(AD0 - y0))), kfabs(val(AD0 - one))) / kfabs(val(AD0)));
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    AJ0[i] = (x0 + y0 + one + 2.0 * p0[i]) * 0.2;
+    delta[i] = (p0[i] - x0) * (p0[i] - y0) * (p0[i] - one);
+    const double a = val(AJ0[i]);
+    Q = kmax(Q, kmax(kmax(kfabs(val(AJ0[i] - x0)), kfabs(val(AJ0[i] - y0))),
+                     kmax(kfabs(val(AJ0[i] - one)), kfabs(val(AJ0[i] - p0[i])))) / kfabs(a));
+  }
+  // The means only shrink by the same lambda-driven map, so one relative
+  // criterion on the initial spread bounds every member: stop when
+  // 4^-m * Q <= 1/150 * (A_m / A_0), checked on the slowest-shrinking mean.
+  Q *= 150.0;
+  T x = x0, y = y0, z = one;
+  T p[NP];
+  T AF = AF0, AD = AD0, AJ[NP], sJ[NP];
+  T sD = T(0.0);
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    p[i] = p0[i];
+    AJ[i] = AJ0[i];
+    sJ[i] = T(0.0);
+  }
+  double f = 1.0, f3 = 1.0;
+  for (int it = 0; it < 16; ++it) {
+    // member-wise criterion Q_rel * |A_0| * 4^-m <= |A_m|
+    bool done = Q * f * kfabs(val(AF0)) <= kfabs(val(AF));
+    if (want_rd) done = done && (Q * f * kfabs(val(AD0)) <= kfabs(val(AD)));
+#pragma unroll
+    for (int i = 0; i < NP; ++i) done = done && (Q * f * kfabs(val(AJ0[i])) <= kfabs(val(AJ[i])));
+    if (done) break;
+    T sx = ksqrt(x), sy = ksqrt(y), sz = ksqrt(z);
+    T lam = sx * (sy + sz) + sy * sz;
+    if (want_rd) sD = sD + f / (sz * (z + lam));
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      T sp = ksqrt(p[i]);
+      T d = (sp + sx) * (sp + sy) * (sp + sz);
+      T e = (delta[i] * f3) / (d * d);
+      sJ[i] = sJ[i] + f * rc_1pe(e) / d;
+      p[i] = (p[i] + lam) * 0.25;
+      AJ[i] = (AJ[i] + lam) * 0.25;
+    }
+    x = (x + lam) * 0.25;
+    y = (y + lam) * 0.25;
+    z = (z + lam) * 0.25;
+    AF = (AF + lam) * 0.25;
+    AD = (AD + lam) * 0.25;
+    f *= 0.25;
+    f3 *= (1.0 / 64.0);
+  }
+  {  // R_F series (DLMF 19.36.1)
+    T X = (AF0 - x0) * f / AF;
+    T Y = (AF0 - y0) * f / AF;
+    T Z = -(X + Y);
+    T E2 = X * Y - Z * Z;
+    T E3 = X * Y * Z;
+    T sr = 1.0 + E2 * (-1.0 / 10.0 + E2 * (1.0 / 24.0 - E2 * (5.0 / 208.0))) +
+           E3 * (1.0 / 14.0 + E3 * (3.0 / 104.0) + E2 * (-3.0 / 44.0 + E2 * (1.0 / 16.0)));
+    out.rf = sr / ksqrt(AF);
+  }
+  if (want_rd) {
+    T X = (AD0 - x0) * f / AD;
+    T Y = (AD0 - y0) * f / AD;
+    T Z = (X + Y) * (-1.0 / 3.0);
+    T XY = X * Y, Z2 = Z * Z;
+    T sr = rj_series(XY - 6.0 * Z2, (3.0 * XY - 8.0 * Z2) * Z, 3.0 * (XY - Z2) * Z2, XY * Z2 * Z);
+    out.rd = f * sr / (AD * ksqrt(AD)) + 3.0 * sD;
+  } else {
+    out.rd = T(0.0);
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    T X = (AJ0[i] - x0) * f / AJ[i];
+    T Y = (AJ0[i] - y0) * f / AJ[i];
+    T Z = (AJ0[i] - one) * f / AJ[i];
+    T P = (X + Y + Z) * (-0.5);
+    T P2 = P * P;
+    T E2 = X * Y + X * Z + Y * Z - 3.0 * P2;
+    T XYZ = X * Y * Z;
+    T sr = rj_series(E2, XYZ + 2.0 * E2 * P + 4.0 * P * P2, (2.0 * XYZ + E2 * P + 3.0 * P * P2) * P, XYZ * P2);
+    T rjp = f * sr / (AJ[i] * ksqrt(AJ[i])) + 6.0 * sJ[i];
+    if (neg[i]) {
+      T xy = x0 * y0;
+      T pq = p0[i] * q[i];
+      T rc = carlson_rc(xy + pq, pq);
+      T fac = 3.0 * ksqrt(xy / (xy + pq));
+      out.rj[i] = ((p0[i] - one) * rjp - 3.0 * out.rf + fac * rc) / (q[i] + one);
+    } else {
+      out.rj[i] = rjp;
+    }
+  }
+}
+
 // Complete integrals K(m), E(m), Pi(n|m) (phi = pi/2: x = 0, y = 1 - m).
 template <class T> KG_INLINE T ellip_K(const T& m) { return carlson_rf(T(0.0), 1.0 - m, T(1.0)); }
 

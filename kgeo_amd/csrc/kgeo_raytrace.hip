@@ -8,8 +8,8 @@
 //                    theta and radial integrals -> slot-major plan [PL_N][npix];
 //                    wave-ballot OR of the image-wide flags the reference's
 //                    numpy code decides globally (any eta > 0, any zero step)
-//   kg_samples       one lane per (pixel, sample chunk): loads its plan slots
-//                    once (coalesced), builds both Jacobi ladders, loops over
+//   kg_theta,        one lane per (pixel, sample chunk): loads its plan slots
+//   kg_radial        once (coalesced), builds its Jacobi ladder, loops over
 //                    its Mino-time samples and stores r, theta, phi, t, sigma
 //                    at [j][npix] -- 64 lanes write 512 contiguous bytes.
 // The work is FP64 VALU (no MFMA): Carlson duplication, Landen ladders and
@@ -106,48 +106,132 @@ __global__ __launch_bounds__(kBlock) void kg_setup(DevParams p, const double* __
   wave_or(gflags, step_zero, GF_ANY_STEP_ZERO);
 }
 
+// ---------------------------------------------------------------------------
+// Sample kernels.  The theta kernel writes theta and parks G_phi, G_t in the
+// phi and t outputs; the radial kernel then writes r and combines
+// phi = I_phi + lambda G_phi, t = I_t + a^2 G_t, sigma = I_sigma + a^2 G_t
+// (kerr_raytracing_ana.py:186-188).  Each kernel runs one loop specialised at
+// compile time for the pixel's motion class / radial case, so a wave only
+// holds the constants of its own case in registers.
+// ---------------------------------------------------------------------------
+struct SampleArgs {
+  const double* plan;
+  const uint32_t* gflags;
+  const double* tau_in;
+  double* tau_out;
+  double* t;
+  double* r;
+  double* th;
+  double* ph;
+  double* sig;
+  uint32_t* status;
+  int chunks;
+};
+
+__device__ inline double tau_at(const DevParams& p, const SampleArgs& a, int64_t j, int64_t o, double stop,
+                                bool any_step_zero) {
+  return a.tau_in ? a.tau_in[o] : mino_tau(j, p.ngeo, stop, any_step_zero);
+}
+
+template <int KIND, bool DO_PHI_T>
+__device__ inline bool theta_loop(const DevParams& p, const SampleArgs& a, const ThetaConsts<double>& tc,
+                                  const JacobiLadder<double>& lad, int64_t i, int64_t j0, int64_t j1, double stop,
+                                  bool any_step_zero) {
+  bool bad = false;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t o = j * p.npix + i;
+    const double tau = tau_at(p, a, j, o, stop, any_step_zero);
+    double th = 0.0, Gph = 0.0, Gt = 0.0;
+    if (KIND != THK_NONE) theta_eval<double, KIND>(tc, lad, tau, DO_PHI_T, th, Gph, Gt);
+    if (a.tau_out) a.tau_out[o] = tau;
+    if (a.th) a.th[o] = th;
+    bad |= !isfinite(th);
+    if (DO_PHI_T) {
+      a.ph[o] = Gph;
+      a.t[o] = Gt;
+    }
+  }
+  return bad;
+}
+
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_samples(DevParams p, const double* __restrict__ plan,
-                                                     const uint32_t* __restrict__ gflags,
-                                                     const double* __restrict__ tau_in, double* __restrict__ tau_out,
-                                                     double* __restrict__ t, double* __restrict__ r,
-                                                     double* __restrict__ th, double* __restrict__ ph,
-                                                     double* __restrict__ sig, uint32_t* __restrict__ status,
-                                                     int chunks) {
+__global__ __launch_bounds__(kBlock) void kg_theta(DevParams p, SampleArgs a) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= p.npix) return;
   const int64_t G = p.ngeo;
-  const int64_t j0 = (G * blockIdx.y) / chunks;
-  const int64_t j1 = (G * (blockIdx.y + 1)) / chunks;
-  double pl[PL_N];
+  const int64_t j0 = (G * blockIdx.y) / a.chunks;
+  const int64_t j1 = (G * (blockIdx.y + 1)) / a.chunks;
+  ThetaConsts<double> tc;
 #pragma unroll
-  for (int s = 0; s < PL_N; ++s) pl[s] = plan[static_cast<int64_t>(s) * p.npix + i];
-  PixelConsts<double> pc;
-  plan_unpack(p.c, pl, pc);
-  const uint32_t gf = gflags[0];
+  for (int s = 0; s < TH_NSLOT; ++s) tc.v[s] = a.plan[static_cast<int64_t>(PL_TH + s) * p.npix + i];
+  const int kind = static_cast<int>(a.plan[static_cast<int64_t>(PL_THK) * p.npix + i]);
+  const bool eta0 = a.plan[static_cast<int64_t>(PL_ETA0) * p.npix + i] != 0.0;
+  const double stop = a.plan[static_cast<int64_t>(PL_STOP) * p.npix + i];
+  const uint32_t gf = a.gflags[0];
   const bool any_eta_pos = (gf & GF_ANY_ETA_POS) != 0;
   const bool any_step_zero = (gf & GF_ANY_STEP_ZERO) != 0;
-  const bool theta_on = !(pc.th.eta_zero && !any_eta_pos);
-  bool bad = (pc.lad_th.capped | pc.lad_r.capped) != 0;
-  bool nonfinite = false;
+  const int k = (eta0 && !any_eta_pos) ? THK_NONE : kind;
+  JacobiLadder<double> lad;
+  jacobi_ladder(tc.v[TH_mp], lad);
+  bool bad;
+  switch (k) {
+    case THK_ORD: bad = theta_loop<THK_ORD, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    case THK_VORT: bad = theta_loop<THK_VORT, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    case THK_ORD_K0: bad = theta_loop<THK_ORD_K0, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    default: bad = theta_loop<THK_NONE, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
+  }
+  if (a.status && (bad || lad.capped)) atomicOr(&a.status[i], (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u));
+}
+
+template <int CASE, bool DO_PHI_T>
+__device__ inline bool radial_loop(const DevParams& p, const SampleArgs& a, const RadialConsts<double>& rc,
+                                   const JacobiLadder<double>& lad, const CombineConsts<double>& cmb, int64_t i,
+                                   int64_t j0, int64_t j1, double stop, bool any_step_zero) {
+  bool bad = false;
   for (int64_t j = j0; j < j1; ++j) {
     const int64_t o = j * p.npix + i;
-    const double tau = tau_in ? tau_in[o] : mino_tau(j, G, pc.stop, any_step_zero);
-    Sample<double> s = eval_sample(pc, tau, DO_PHI_T, theta_on);
-    if (tau_out) tau_out[o] = tau;
-    if (r) r[o] = s.r;
-    if (th) th[o] = s.th;
-    nonfinite |= !(isfinite(s.r) && isfinite(s.th));
+    const double tau = tau_at(p, a, j, o, stop, any_step_zero);
+    RadialOut<double> ro = radial_eval<double, CASE>(rc, lad, tau, DO_PHI_T);
+    if (a.r) a.r[o] = ro.r;
+    bad |= !isfinite(ro.r);
     if (DO_PHI_T) {
-      if (t) t[o] = s.t;
-      if (ph) ph[o] = s.ph;
-      if (sig) sig[o] = s.sig;
-      nonfinite |= !(isfinite(s.t) && isfinite(s.ph) && isfinite(s.sig));
+      const double Gph = a.ph[o];
+      const double Gt = a.t[o];
+      Sample<double> s;
+      combine(cmb, tau, ro, 0.0, Gph, Gt, s);
+      a.t[o] = s.t;
+      a.ph[o] = s.ph;
+      if (a.sig) a.sig[o] = s.sig;
+      bad |= !(isfinite(s.t) && isfinite(s.ph) && isfinite(s.sig));
     }
   }
-  if (status && (nonfinite || bad)) {
-    atomicOr(&status[i], (nonfinite ? ST_NONFINITE : 0u) | (bad ? ST_LADDER_CAP : 0u));
+  return bad;
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_radial(DevParams p, SampleArgs a) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  const int64_t G = p.ngeo;
+  const int64_t j0 = (G * blockIdx.y) / a.chunks;
+  const int64_t j1 = (G * (blockIdx.y + 1)) / a.chunks;
+  RadialConsts<double> rc;
+#pragma unroll
+  for (int s = 0; s < R_NSLOT; ++s) rc.v[s] = a.plan[static_cast<int64_t>(PL_R + s) * p.npix + i];
+  rc.rcase = static_cast<int>(a.plan[static_cast<int64_t>(PL_RCASE) * p.npix + i]);
+  const double lam = a.plan[static_cast<int64_t>(PL_LAM) * p.npix + i];
+  const double stop = a.plan[static_cast<int64_t>(PL_STOP) * p.npix + i];
+  const bool any_step_zero = (a.gflags[0] & GF_ANY_STEP_ZERO) != 0;
+  const CombineConsts<double> cmb = combine_consts(p.c, lam);
+  JacobiLadder<double> lad;
+  jacobi_ladder(rc.v[R_k], lad);
+  bool bad;
+  switch (rc.rcase) {
+    case 4: bad = radial_loop<4, DO_PHI_T>(p, a, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
+    case 3: bad = radial_loop<3, DO_PHI_T>(p, a, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
+    default: bad = radial_loop<2, DO_PHI_T>(p, a, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
   }
+  if (a.status && (bad || lad.capped)) atomicOr(&a.status[i], (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u));
 }
 
 __global__ __launch_bounds__(kBlock) void kg_ellip_pi(const double* __restrict__ n, const double* __restrict__ phi,
@@ -258,6 +342,8 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
                double* th, double* ph, double* sig, uint32_t* status, const void* ws, hipStream_t st) {
   DevParams d = make_dev(p, ngeo);
   if (d.npix == 0 || ngeo <= 0) return KGEO_OK;
+  if (d.do_phi_t && (!t || !ph))
+    return fail(KGEO_E_ARG, "t and ph outputs are required with KGEO_DO_PHI_T%s%lld");
   const uint32_t* gflags = static_cast<const uint32_t*>(ws);
   const double* plan = reinterpret_cast<const double*>(static_cast<const char*>(ws) + kFlagBytes);
   const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
@@ -265,14 +351,19 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   if (chunks <= 0) chunks = pick_chunks(d.npix, ngeo);
   if (chunks > ngeo) chunks = static_cast<int>(ngeo);
   dim3 grid(nb, static_cast<unsigned>(chunks));
+  SampleArgs a{plan, gflags, tau_in, tau, t, r, th, ph, sig, status, chunks};
   if (d.do_phi_t) {
-    hipLaunchKernelGGL(kg_samples<true>, grid, dim3(kBlock), 0, st, d, plan, gflags, tau_in, tau, t, r, th, ph, sig,
-                       status, chunks);
+    hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);
+    int rc = check_launch("kg_theta");
+    if (rc) return rc;
+    hipLaunchKernelGGL(kg_radial<true>, grid, dim3(kBlock), 0, st, d, a);
   } else {
-    hipLaunchKernelGGL(kg_samples<false>, grid, dim3(kBlock), 0, st, d, plan, gflags, tau_in, tau, t, r, th, ph, sig,
-                       status, chunks);
+    hipLaunchKernelGGL(kg_theta<false>, grid, dim3(kBlock), 0, st, d, a);
+    int rc = check_launch("kg_theta");
+    if (rc) return rc;
+    hipLaunchKernelGGL(kg_radial<false>, grid, dim3(kBlock), 0, st, d, a);
   }
-  return check_launch("kg_samples");
+  return check_launch("kg_radial");
 }
 
 }  // namespace

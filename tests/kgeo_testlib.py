@@ -168,3 +168,75 @@ class Host:
                               self._p(nm, ctypes.c_int32))
         outs["nmax_eq"] = nm
         return outs
+
+
+# --- equatorial image workloads ---------------------------------------------
+C3 = dict(a=0.27118414089647946,  # np.random.default_rng(0).uniform(-0.99, 0.99)
+          th_deg=60.0, r_o=1.0e4, window=12.0, n=1024, mbar_max=2, p=2.0)
+C5 = dict(a=0.7, th_deg=40.0, p=2.0, target=(0.5, 60.0, 2.5), r_o=1.0e4, window=12.0, n=256, mbar_max=2,
+          fd_step=1.0e-6, crit_dist=1.0e-3)
+
+
+def host_equatorial(host, a, r_o, th, A, B, mbar_max=2, p=2.0):
+    n = len(A)
+    M = mbar_max + 1
+    out = dict(image=np.empty(n), I=np.empty(M * n), g=np.empty(M * n), r_s=np.empty(M * n), Ir=np.empty(M * n),
+               Imax=np.empty(n), Nmax=np.empty(n, np.int32), zmask=np.empty(n, np.int32))
+    P = Host._p
+    host.lib.kgh_equatorial(P(Host.scalars(a, r_o, th)), P(np.ascontiguousarray(A, float)),
+                            P(np.ascontiguousarray(B, float)), ctypes.c_int64(n), ctypes.c_int(mbar_max),
+                            ctypes.c_double(p), P(out["image"]), P(out["I"]), P(out["g"]), P(out["r_s"]),
+                            P(out["Ir"]), P(out["Imax"]), P(out["Nmax"], ctypes.c_int32),
+                            P(out["zmask"], ctypes.c_int32))
+    for k in ("I", "g", "r_s", "Ir"):
+        out[k] = out[k].reshape(M, n)
+    return out
+
+
+def host_equatorial_jac(host, a, th, r_o, p, A, B, mbar_max=2):
+    n = len(A)
+    img = np.empty(n)
+    jac = np.empty(3 * n)
+    P = Host._p
+    host.lib.kgh_equatorial_jac(ctypes.c_double(a), ctypes.c_double(th), ctypes.c_double(r_o), ctypes.c_double(p),
+                                P(np.ascontiguousarray(A, float)), P(np.ascontiguousarray(B, float)),
+                                ctypes.c_int64(n), ctypes.c_int(mbar_max), P(img), P(jac))
+    return img, jac.reshape(3, n)
+
+
+def image_errors(x, y):
+    """Relative error of image values; exact zeros (masked pixels) compared absolutely."""
+    x = np.asarray(x, float)
+    y = np.asarray(y, float)
+    return np.where(y == 0, np.abs(x), np.abs(x - y) / np.maximum(np.abs(y), 1e-300))
+
+
+def fd_loss_gradient(theta, target, keep, A, B, r_o, mbar_max, h):
+    """Central finite differences (step h) of ||image(theta) - target||_2 over
+    `keep`, with the oracle's image.  Also returns the per-parameter integer
+    fingerprints (N_max, masks) used to detect discontinuities."""
+    L = lambda im: np.sqrt(np.sum(((im - target) * keep) ** 2))
+    grad = np.zeros(3)
+    for k in range(3):
+        vals = []
+        for s in (1.0, -1.0):
+            q = np.array(theta, float)
+            q[k] += s * h
+            im, _ = O.equatorial_image(q[0], r_o, q[1], A, B, mbar_max, q[2])
+            vals.append(L(im))
+        grad[k] = (vals[0] - vals[1]) / (2 * h)
+    return grad
+
+
+def discontinuity_mask(theta, A, B, r_o, mbar_max, h):
+    """Pixels whose N_max or subring masks change within the FD stencil."""
+    _, base = O.equatorial_image(theta[0], r_o, theta[1], A, B, mbar_max, theta[2])
+    disc = np.zeros(len(A), bool)
+    for k in range(3):
+        for s in (1.0, -1.0):
+            q = np.array(theta, float)
+            q[k] += s * h
+            _, parts = O.equatorial_image(q[0], r_o, q[1], A, B, mbar_max, q[2])
+            for m in range(mbar_max + 1):
+                disc |= (parts[m]["Nmax"] != base[m]["Nmax"]) | (parts[m]["zeromask"] != base[m]["zeromask"])
+    return disc
