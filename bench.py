@@ -133,18 +133,163 @@ def read_traffic():
     return None
 
 
+def _events(n):
+    import torch
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def bench_batch(args):
+    """Batched survey: 256 seeded images (spin, inclination, resolution,
+    ngeo) in one persistent graph launch, against the same survey as 256
+    sequential plain kgeo_raytrace_ana calls into the same arenas."""
+    import torch
+    from kgeo_amd import autotune
+    from kgeo_amd.batch import KEYS, BatchSurvey, layout, run_sequential, survey_draw, survey_images, survey_size
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    draw = survey_draw(256, seed=0)
+    images = survey_images(draw, window=WINDOW)
+    pix, samples = survey_size(images)
+    _, out_off = layout(images, r_o=R_O)
+    log(f"batch survey: {len(images)} images, {pix} px, {samples} samples ({samples * 32 / 1e9:.1f} GB of r,th,ph,t)")
+    arena = {k: torch.empty(int(out_off[-1]), device=dev, dtype=torch.float64) for k in KEYS}
+    npix = np.array([len(im["alpha"]) for im in images])
+    ngeo = np.array([im["ngeo"] for im in images])
+    key = autotune.batch_key(npix, ngeo)
+    choice = autotune.lookup(key)
+    tune_ms = None
+    if choice is None or args.retune:
+        sub = images[::8]
+        log(f"autotuning on {len(sub)} of the images (key {key})")
+        t0 = time.perf_counter()
+        choice, timings = autotune.tune_batch(sub, key=key, arena=arena, log=log)
+        tune_ms = (time.perf_counter() - t0) * 1e3
+        log(f"autotune choice {choice} in {tune_ms / 1e3:.1f} s")
+    sv = BatchSurvey(images, r_o=R_O, tuning=choice, arena=arena)
+    log(f"planned: {sv.items} work items, tuning {sv.tuning}")
+    for _ in range(args.warmup):
+        sv.run()
+    sv.stream.synchronize()
+    ev = _events(2 * args.steps)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[2 * i].record(sv.stream)
+        sv.run()
+        ev[2 * i + 1].record(sv.stream)
+    sv.stream.synchronize()
+    wall_b = (time.perf_counter() - t0) / args.steps
+    gpu_b = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)])) / 1e3
+    done, total = sv.progress()
+    log(f"batched: {wall_b * 1e3:.1f} ms/survey (events {gpu_b * 1e3:.1f} ms) -> {samples / wall_b:.3e} samples/s")
+    # keep a few images' batched outputs for the accuracy comparison
+    keep = list(range(0, len(images), 37))
+    saved = {k: [sv.image(i)[key_].cpu() for key_ in KEYS] for k, i in enumerate(keep)}
+    labels_b = {k: getattr(sv, k).cpu() for k in ("rclass", "thclass", "nmax_eq", "status")}
+    dev_labels = {k: torch.empty_like(getattr(sv, k)) for k in ("rclass", "thclass", "nmax_eq", "tau_tot", "status")}
+    ws = run_sequential(images, arena, out_off, r_o=R_O, labels=dev_labels)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_sequential(images, arena, out_off, r_o=R_O, workspace=ws, labels=dev_labels)
+    torch.cuda.synchronize()
+    wall_s = (time.perf_counter() - t0) / args.steps
+    log(f"sequential plain launches: {wall_s * 1e3:.1f} ms/survey -> {samples / wall_s:.3e} samples/s")
+    same = True
+    for k, i in enumerate(keep):
+        im = sv.image(i)
+        for j, key_ in enumerate(KEYS):
+            a_, b_ = saved[k][j], im[key_].cpu()
+            same &= bool(torch.equal(torch.nan_to_num(a_, nan=1.5e308), torch.nan_to_num(b_, nan=1.5e308)))
+    for k in labels_b:
+        same &= bool(torch.equal(labels_b[k], dev_labels[k].cpu()))
+    line = {"metric": "batched survey geodesic samples/s (256 images, one persistent hipGraph launch)",
+            "value": samples / wall_b, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall_b * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "256-image survey: a~U(-0.99,0.99), th_o~U(1,89) deg, res from "
+                                   "{64,128,256,512}^2, ngeo~U{50..400}, default_rng(0); alpha,beta in [-8,8], "
+                                   "r_o=1e4; writes r, theta, phi, t",
+                       "images": len(images), "pixels": int(pix), "samples": int(samples),
+                       "tuning": {"samples_per_item": sv.tuning[0], "waves_per_cu": sv.tuning[1]},
+                       "work_items": sv.items, "autotune_key": key, "autotune_ms": tune_ms},
+            "batched_event_ms": gpu_b * 1e3,
+            "sequential_plain": {"value": samples / wall_s, "ms_per_survey": wall_s * 1e3,
+                                 "launches": 4 * len(images)},
+            "speedup_vs_sequential": wall_s / wall_b,
+            "outputs_identical_to_plain": same,
+            "all_items_done": bool(np.array_equal(done, total))}
+    print(json.dumps(line), flush=True)
+
+
+def bench_equatorial(args):
+    """Equatorial-emission image 1024^2 (a = first draw of default_rng(0)
+    over [-0.99, 0.99], th_o = 60 deg, mbar 0..2, p = 2), and the gradient
+    image (vector-Jacobian product over (a, th_o, p)) at 256^2, a = 0.7,
+    th_o = 40 deg, against the plain forward image at that size."""
+    import torch
+    from kgeo_amd import _lib
+    from kgeo_amd.equatorial import equatorial_image_device, equatorial_image_vjp_device
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    res = {}
+    for name, (a, thd, n, w) in {"image_1024": (0.27118414089647946, 60.0, 1024, 12.0),
+                                 "image_256": (0.7, 40.0, 256, 12.0)}.items():
+        al = np.linspace(-w, w, n)
+        A, B = np.meshgrid(al, al)
+        A = torch.from_numpy(A.flatten()).to(dev)
+        B = torch.from_numpy(B.flatten()).to(dev)
+        th = thd * np.pi / 180.0
+        for _ in range(args.warmup):
+            equatorial_image_device(a, R_O, th, A, B, want_parts=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            equatorial_image_device(a, R_O, th, A, B, want_parts=False)
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / args.steps
+        log(f"{name}: {res[name] * 1e3:.3f} ms/image")
+        if n == 256:
+            for _ in range(args.warmup):
+                equatorial_image_vjp_device(a, R_O, th, A, B)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                equatorial_image_vjp_device(a, R_O, th, A, B)
+            torch.cuda.synchronize()
+            res["grad_256"] = (time.perf_counter() - t0) / args.steps
+            log(f"grad_256: {res['grad_256'] * 1e3:.3f} ms/grad-image")
+    line = {"metric": "equatorial images/s at 1024^2; grad-images/s at 256^2",
+            "value": 1.0 / res["grad_256"], "unit": "grad-images/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["grad_256"] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "d image / d (a, th_o, p) at 256^2, a=0.7, th_o=40 deg, p=2, mbar 0..2, "
+                                   "alpha,beta in [-12,12]; forward images at 1024^2 (a=0.2712, th_o=60 deg) and "
+                                   "256^2"},
+            "images_per_s_1024": 1.0 / res["image_1024"], "images_per_s_256": 1.0 / res["image_256"],
+            "grad_over_forward_time_256": res["grad_256"] / res["image_256"]}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial"), default="raytrace")
+    ap.add_argument("--retune", action="store_true", help="batch mode: re-run the autotuner")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
               "the raytracer shards over pixels with no collectives, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "equatorial":
+        return bench_equatorial(args)
 
     import torch
     from kgeo_amd import _lib

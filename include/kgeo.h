@@ -173,6 +173,51 @@ int kgeo_equatorial_image_vjp(const kgeo_ray_params* p, const double* alpha, con
                               size_t workspace_bytes, void* stream);
 const char* kgeo_equatorial_last_error(void);
 
+/*
+ * No counterpart in the reference (its callers loop over raytrace_ana, one
+ * image per call).  Batched survey: nimg images, each with its own
+ * kgeo_ray_params (a, r_o, th_o, npix, ngeo, flags), traced by one
+ * persistent launch captured in a HIP graph.  Each image's results equal a
+ * kgeo_raytrace_ana call on that image alone (same kernels' code): the
+ * image-wide predicates (any eta > 0, any zero grid step) are per image.
+ *
+ * Layout (kgeo_batch_layout gives the offsets, nimg + 1 entries each):
+ *   alpha, beta, rclass, thclass, nmax_eq, tau_tot, status: concatenated
+ *     per-pixel arrays, image k at pix_off[k];
+ *   t, r, th, ph: one arena, image k's [ngeo_k][npix_k] block at out_off[k].
+ * tau and sigma are not stored (tau is the image's linspace grid; sigma can
+ * be recomputed with kgeo_coords_at_tau).  t and ph may be NULL only if no
+ * image sets KGEO_DO_PHI_T; rclass..status are optional.
+ *
+ * kgeo_batch_plan uploads the image descriptors, records the graph on
+ * `stream` (must not be NULL) and returns a handle; it runs no kernels.
+ * kgeo_batch_run replays the whole survey on any stream.  The handle keeps
+ * the pointers it was given: buffers and workspace must outlive it, and runs
+ * that overlap in time on one handle race.  kgeo_batch_progress copies the
+ * per-image completed work-item counts of the last run to the host (after
+ * the run's stream is synchronised), so a caller can tell finished images
+ * from unfinished ones.
+ */
+typedef struct kgeo_batch_tuning {
+  int32_t samples_per_item; /* Mino-time samples per work item; a work item is 64 pixels x this */
+  int32_t waves_per_cu;     /* persistent grid: CUs x this waves (1..64) */
+  int32_t reserved[2];
+} kgeo_batch_tuning;
+
+typedef struct kgeo_batch kgeo_batch;
+
+size_t kgeo_batch_workspace_bytes(const kgeo_ray_params* imgs, int32_t nimg);
+int kgeo_batch_layout(const kgeo_ray_params* imgs, int32_t nimg, int64_t* pix_off, int64_t* out_off);
+int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_tuning* tune, const double* alpha,
+                    const double* beta, double* t, double* r, double* th, double* ph, int8_t* rclass,
+                    int8_t* thclass, int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* workspace,
+                    size_t workspace_bytes, void* stream, kgeo_batch** out);
+int kgeo_batch_run(kgeo_batch* b, void* stream);
+int64_t kgeo_batch_items(const kgeo_batch* b);
+int kgeo_batch_progress(const kgeo_batch* b, uint32_t* done_items, uint32_t* total_items);
+int kgeo_batch_destroy(kgeo_batch* b);
+const char* kgeo_batch_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,13 @@ class RayParams(ctypes.Structure):
     ]
 
 
+class BatchTuning(ctypes.Structure):
+    """kgeo_batch_tuning (include/kgeo.h)."""
+
+    _fields_ = [("samples_per_item", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
+
+
 class KgeoError(RuntimeError):
     pass
 
@@ -68,6 +75,17 @@ _PROTOS = {
     "kgeo_equatorial_last_error": (ctypes.c_char_p, []),
     "kgeo_crossings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
                        + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "kgeo_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(RayParams), ctypes.c_int32]),
+    "kgeo_batch_layout": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    "kgeo_batch_plan": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_int32, ctypes.POINTER(BatchTuning)]
+                        + [ctypes.c_void_p] * 12 + [ctypes.c_size_t, ctypes.c_void_p,
+                                                    ctypes.POINTER(ctypes.c_void_p)]),
+    "kgeo_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "kgeo_batch_items": (ctypes.c_int64, [ctypes.c_void_p]),
+    "kgeo_batch_progress": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "kgeo_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "kgeo_batch_last_error": (ctypes.c_char_p, []),
 }
 
 
@@ -98,7 +116,12 @@ def load():
 def check(rc, what):
     if rc != 0:
         lib = load()
-        err = lib.kgeo_equatorial_last_error if what.startswith("kgeo_equatorial") else lib.kgeo_last_error
+        if what.startswith("kgeo_equatorial"):
+            err = lib.kgeo_equatorial_last_error
+        elif what.startswith("kgeo_batch"):
+            err = lib.kgeo_batch_last_error
+        else:
+            err = lib.kgeo_last_error
         msg = err().decode(errors="replace")
         raise KgeoError(f"{what} failed ({rc}): {msg}")
 

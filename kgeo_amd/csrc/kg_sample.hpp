@@ -1,0 +1,173 @@
+// Per-pixel sample loops shared by the plain kernels (kgeo_raytrace.hip) and
+// the persistent batch kernel (kgeo_batch.hip), so both run the same code.
+//
+// theta_pixel: loads the theta slots of one pixel's plan, builds the theta
+//   Jacobi ladder and, for samples j0..j1-1, writes theta and parks
+//   G_phi -> ph, G_t -> t (kerr_raytracing_ana.py:186-188 needs them there).
+// radial_pixel: loads the radial slots, builds the radial ladder, writes r and
+//   combines phi = I_phi + lambda G_phi, t = I_t + a^2 G_t,
+//   sigma = I_sigma + a^2 G_t (kerr_raytracing_ana.py:186-188).
+// Each runs one loop specialised at compile time for the pixel's motion class
+// or radial case, so a wave only keeps its own case's constants live.  The
+// plan is slot-major: slot s of pixel p is plan[s * stride + p].
+
+#pragma once
+
+#include "kg_geodesic.hpp"
+
+namespace kg {
+
+enum : uint32_t { GF_ANY_ETA_POS = 1u, GF_ANY_STEP_ZERO = 2u };
+
+struct SampleOut {
+  const double* tau_in;  // optional [ngeo][npix] Mino times replacing the grid
+  double* tau_out;       // optional
+  double* t;
+  double* r;
+  double* th;
+  double* ph;
+  double* sig;           // optional
+  int64_t npix;          // row stride of the [ngeo][npix] arrays
+  int64_t ngeo;
+};
+
+__device__ inline double sample_tau(const SampleOut& o, int64_t j, int64_t idx, double stop, bool any_step_zero) {
+  return o.tau_in ? o.tau_in[idx] : mino_tau(j, o.ngeo, stop, any_step_zero);
+}
+
+template <int KIND, bool DO_PHI_T>
+__device__ inline bool theta_loop(const SampleOut& o, const ThetaConsts<double>& tc, const JacobiLadder<double>& lad,
+                                  int64_t i, int64_t j0, int64_t j1, double stop, bool any_step_zero) {
+  bool bad = false;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t idx = j * o.npix + i;
+    const double tau = sample_tau(o, j, idx, stop, any_step_zero);
+    double th = 0.0, Gph = 0.0, Gt = 0.0;
+    if (KIND != THK_NONE) theta_eval<double, KIND>(tc, lad, tau, DO_PHI_T, th, Gph, Gt);
+    if (o.tau_out) o.tau_out[idx] = tau;
+    if (o.th) o.th[idx] = th;
+    bad |= !isfinite(th);
+    if (DO_PHI_T) {
+      o.ph[idx] = Gph;
+      o.t[idx] = Gt;
+    }
+  }
+  return bad;
+}
+
+template <int CASE, bool DO_PHI_T>
+__device__ inline bool radial_loop(const SampleOut& o, const RadialConsts<double>& rc, const JacobiLadder<double>& lad,
+                                   const CombineConsts<double>& cmb, int64_t i, int64_t j0, int64_t j1, double stop,
+                                   bool any_step_zero) {
+  bool bad = false;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t idx = j * o.npix + i;
+    const double tau = sample_tau(o, j, idx, stop, any_step_zero);
+    RadialOut<double> ro = radial_eval<double, CASE>(rc, lad, tau, DO_PHI_T);
+    if (o.r) o.r[idx] = ro.r;
+    bad |= !isfinite(ro.r);
+    if (DO_PHI_T) {
+      const double Gph = o.ph[idx];
+      const double Gt = o.t[idx];
+      Sample<double> s;
+      combine(cmb, tau, ro, 0.0, Gph, Gt, s);
+      o.t[idx] = s.t;
+      o.ph[idx] = s.ph;
+      if (o.sig) o.sig[idx] = s.sig;
+      bad |= !(isfinite(s.t) && isfinite(s.ph) && isfinite(s.sig));
+    }
+  }
+  return bad;
+}
+
+__device__ inline uint32_t loop_status(bool bad, const JacobiLadder<double>& lad) {
+  return (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u);
+}
+
+// Theta part of samples j0..j1-1 of output pixel i, whose plan column is p.
+template <bool DO_PHI_T>
+__device__ inline uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                       const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
+  ThetaConsts<double> tc;
+#pragma unroll
+  for (int s = 0; s < TH_NSLOT; ++s) tc.v[s] = plan[static_cast<int64_t>(PL_TH + s) * stride + p];
+  const int kind = static_cast<int>(plan[static_cast<int64_t>(PL_THK) * stride + p]);
+  const bool eta0 = plan[static_cast<int64_t>(PL_ETA0) * stride + p] != 0.0;
+  const double stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
+  const bool any_eta_pos = (gflags & GF_ANY_ETA_POS) != 0;
+  const bool any_step_zero = (gflags & GF_ANY_STEP_ZERO) != 0;
+  // eta == 0 pixels take theta = 0 when no pixel of the image has eta > 0
+  const int k = (eta0 && !any_eta_pos) ? THK_NONE : kind;
+  JacobiLadder<double> lad;
+  jacobi_ladder(tc.v[TH_mp], lad);
+  bool bad;
+  switch (k) {
+    case THK_ORD: bad = theta_loop<THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    case THK_VORT: bad = theta_loop<THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    case THK_ORD_K0: bad = theta_loop<THK_ORD_K0, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    default: bad = theta_loop<THK_NONE, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+  }
+  return loop_status(bad, lad);
+}
+
+// Radial part (and the phi, t, sigma combination) of the same samples.
+template <bool DO_PHI_T>
+__device__ inline uint32_t radial_pixel(const RayScalars<double>& c, const double* __restrict__ plan, int64_t stride,
+                                        int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j0,
+                                        int64_t j1) {
+  RadialConsts<double> rc;
+#pragma unroll
+  for (int s = 0; s < R_NSLOT; ++s) rc.v[s] = plan[static_cast<int64_t>(PL_R + s) * stride + p];
+  rc.rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
+  const double lam = plan[static_cast<int64_t>(PL_LAM) * stride + p];
+  const double stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
+  const bool any_step_zero = (gflags & GF_ANY_STEP_ZERO) != 0;
+  const CombineConsts<double> cmb = combine_consts(c, lam);
+  JacobiLadder<double> lad;
+  jacobi_ladder(rc.v[R_k], lad);
+  bool bad;
+  switch (rc.rcase) {
+    case 4: bad = radial_loop<4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
+    case 3: bad = radial_loop<3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
+    default: bad = radial_loop<2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
+  }
+  return loop_status(bad, lad);
+}
+
+// Per-pixel part of the plan kernels: the plan column, labels and the two
+// image-wide predicates the reference's numpy code evaluates over all pixels.
+struct SetupOut {
+  bool eta_pos;
+  bool step_zero;
+};
+
+__device__ inline SetupOut setup_pixel(const RayScalars<double>& c, int64_t ngeo, bool do_phi_t, double al, double be,
+                                       double* __restrict__ plan, int64_t stride, int64_t p, int8_t* rclass,
+                                       int8_t* thclass, int32_t* nmax_eq, double* tau_tot, uint32_t* status,
+                                       int64_t i) {
+  double pl[PL_N];
+  PixelSetup<double> px = plan_pixel(c, al, be, do_phi_t, pl);
+#pragma unroll
+  for (int s = 0; s < PL_N; ++s) plan[static_cast<int64_t>(s) * stride + p] = pl[s];
+  SetupOut so;
+  so.eta_pos = px.eta > 0.0;
+  const double stop = pl[PL_STOP];
+  so.step_zero = (ngeo > 1) && (stop / static_cast<double>(ngeo - 1) == 0.0);
+  if (rclass) rclass[i] = static_cast<int8_t>(px.rclass);
+  if (thclass) thclass[i] = static_cast<int8_t>(px.thclass);
+  if (tau_tot) tau_tot[i] = px.tau_tot;
+  if (nmax_eq) {
+    // the last grid value is `stop` for ngeo > 1 in both linspace branches
+    const double tl = (ngeo > 1) ? stop : 0.0 * stop;
+    nmax_eq[i] = nmax_equatorial_at(c, al, be, tl);
+  }
+  if (status) status[i] = px.status;
+  return so;
+}
+
+__device__ inline void wave_or(uint32_t* word, bool pred, uint32_t bit) {
+  const unsigned long long b = __ballot(pred ? 1 : 0);
+  if (b != 0ull && __lane_id() == 0) atomicOr(word, bit);
+}
+
+}  // namespace kg

@@ -23,7 +23,7 @@
 #include <cstring>
 
 #include "../../include/kgeo.h"
-#include "kg_geodesic.hpp"
+#include "kg_sample.hpp"
 
 using namespace kg;
 
@@ -44,7 +44,6 @@ int check_launch(const char* what) {
 
 constexpr int kBlock = 256;
 constexpr size_t kFlagBytes = 256;
-enum : uint32_t { GF_ANY_ETA_POS = 1u, GF_ANY_STEP_ZERO = 2u };
 
 struct DevParams {
   RayScalars<double> c;
@@ -67,10 +66,6 @@ DevParams make_dev(const kgeo_ray_params* p, int64_t ngeo) {
   return d;
 }
 
-__device__ inline void wave_or(uint32_t* word, bool pred, uint32_t bit) {
-  const unsigned long long b = __ballot(pred ? 1 : 0);
-  if (b != 0ull && __lane_id() == 0) atomicOr(word, bit);
-}
 
 __global__ void kg_clear_flags(uint32_t* flags) {
   if (threadIdx.x < kFlagBytes / sizeof(uint32_t)) flags[threadIdx.x] = 0u;
@@ -82,77 +77,26 @@ __global__ __launch_bounds__(kBlock) void kg_setup(DevParams p, const double* __
                                                    int8_t* __restrict__ thclass, int32_t* __restrict__ nmax_eq,
                                                    double* __restrict__ tau_tot, uint32_t* __restrict__ status) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  bool eta_pos = false, step_zero = false;
-  if (i < p.npix) {
-    double pl[PL_N];
-    const double al = alpha[i], be = beta[i];
-    PixelSetup<double> px = plan_pixel(p.c, al, be, p.do_phi_t != 0, pl);
-#pragma unroll
-    for (int s = 0; s < PL_N; ++s) plan[static_cast<int64_t>(s) * p.npix + i] = pl[s];
-    eta_pos = px.eta > 0.0;
-    const double stop = pl[PL_STOP];
-    step_zero = (p.ngeo > 1) && (stop / static_cast<double>(p.ngeo - 1) == 0.0);
-    if (rclass) rclass[i] = static_cast<int8_t>(px.rclass);
-    if (thclass) thclass[i] = static_cast<int8_t>(px.thclass);
-    if (tau_tot) tau_tot[i] = px.tau_tot;
-    if (nmax_eq) {
-      // the last grid value is `stop` for ngeo > 1 in both linspace branches
-      const double tl = (p.ngeo > 1) ? stop : 0.0 * stop;
-      nmax_eq[i] = nmax_equatorial_at(p.c, al, be, tl);
-    }
-    if (status) status[i] = px.status;
-  }
-  wave_or(gflags, eta_pos, GF_ANY_ETA_POS);
-  wave_or(gflags, step_zero, GF_ANY_STEP_ZERO);
+  SetupOut so{false, false};
+  if (i < p.npix)
+    so = setup_pixel(p.c, p.ngeo, p.do_phi_t != 0, alpha[i], beta[i], plan, p.npix, i, rclass, thclass, nmax_eq,
+                     tau_tot, status, i);
+  wave_or(gflags, so.eta_pos, GF_ANY_ETA_POS);
+  wave_or(gflags, so.step_zero, GF_ANY_STEP_ZERO);
 }
 
 // ---------------------------------------------------------------------------
-// Sample kernels.  The theta kernel writes theta and parks G_phi, G_t in the
-// phi and t outputs; the radial kernel then writes r and combines
-// phi = I_phi + lambda G_phi, t = I_t + a^2 G_t, sigma = I_sigma + a^2 G_t
-// (kerr_raytracing_ana.py:186-188).  Each kernel runs one loop specialised at
-// compile time for the pixel's motion class / radial case, so a wave only
-// holds the constants of its own case in registers.
+// Sample kernels: grid (pixel blocks, sample chunks).  kg_theta writes theta
+// and parks G_phi, G_t in ph, t; kg_radial writes r and combines phi, t,
+// sigma (kg_sample.hpp).
 // ---------------------------------------------------------------------------
 struct SampleArgs {
   const double* plan;
   const uint32_t* gflags;
-  const double* tau_in;
-  double* tau_out;
-  double* t;
-  double* r;
-  double* th;
-  double* ph;
-  double* sig;
+  SampleOut out;
   uint32_t* status;
   int chunks;
 };
-
-__device__ inline double tau_at(const DevParams& p, const SampleArgs& a, int64_t j, int64_t o, double stop,
-                                bool any_step_zero) {
-  return a.tau_in ? a.tau_in[o] : mino_tau(j, p.ngeo, stop, any_step_zero);
-}
-
-template <int KIND, bool DO_PHI_T>
-__device__ inline bool theta_loop(const DevParams& p, const SampleArgs& a, const ThetaConsts<double>& tc,
-                                  const JacobiLadder<double>& lad, int64_t i, int64_t j0, int64_t j1, double stop,
-                                  bool any_step_zero) {
-  bool bad = false;
-  for (int64_t j = j0; j < j1; ++j) {
-    const int64_t o = j * p.npix + i;
-    const double tau = tau_at(p, a, j, o, stop, any_step_zero);
-    double th = 0.0, Gph = 0.0, Gt = 0.0;
-    if (KIND != THK_NONE) theta_eval<double, KIND>(tc, lad, tau, DO_PHI_T, th, Gph, Gt);
-    if (a.tau_out) a.tau_out[o] = tau;
-    if (a.th) a.th[o] = th;
-    bad |= !isfinite(th);
-    if (DO_PHI_T) {
-      a.ph[o] = Gph;
-      a.t[o] = Gt;
-    }
-  }
-  return bad;
-}
 
 template <bool DO_PHI_T>
 __global__ __launch_bounds__(kBlock) void kg_theta(DevParams p, SampleArgs a) {
@@ -161,51 +105,8 @@ __global__ __launch_bounds__(kBlock) void kg_theta(DevParams p, SampleArgs a) {
   const int64_t G = p.ngeo;
   const int64_t j0 = (G * blockIdx.y) / a.chunks;
   const int64_t j1 = (G * (blockIdx.y + 1)) / a.chunks;
-  ThetaConsts<double> tc;
-#pragma unroll
-  for (int s = 0; s < TH_NSLOT; ++s) tc.v[s] = a.plan[static_cast<int64_t>(PL_TH + s) * p.npix + i];
-  const int kind = static_cast<int>(a.plan[static_cast<int64_t>(PL_THK) * p.npix + i]);
-  const bool eta0 = a.plan[static_cast<int64_t>(PL_ETA0) * p.npix + i] != 0.0;
-  const double stop = a.plan[static_cast<int64_t>(PL_STOP) * p.npix + i];
-  const uint32_t gf = a.gflags[0];
-  const bool any_eta_pos = (gf & GF_ANY_ETA_POS) != 0;
-  const bool any_step_zero = (gf & GF_ANY_STEP_ZERO) != 0;
-  const int k = (eta0 && !any_eta_pos) ? THK_NONE : kind;
-  JacobiLadder<double> lad;
-  jacobi_ladder(tc.v[TH_mp], lad);
-  bool bad;
-  switch (k) {
-    case THK_ORD: bad = theta_loop<THK_ORD, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
-    case THK_VORT: bad = theta_loop<THK_VORT, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
-    case THK_ORD_K0: bad = theta_loop<THK_ORD_K0, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
-    default: bad = theta_loop<THK_NONE, DO_PHI_T>(p, a, tc, lad, i, j0, j1, stop, any_step_zero); break;
-  }
-  if (a.status && (bad || lad.capped)) atomicOr(&a.status[i], (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u));
-}
-
-template <int CASE, bool DO_PHI_T>
-__device__ inline bool radial_loop(const DevParams& p, const SampleArgs& a, const RadialConsts<double>& rc,
-                                   const JacobiLadder<double>& lad, const CombineConsts<double>& cmb, int64_t i,
-                                   int64_t j0, int64_t j1, double stop, bool any_step_zero) {
-  bool bad = false;
-  for (int64_t j = j0; j < j1; ++j) {
-    const int64_t o = j * p.npix + i;
-    const double tau = tau_at(p, a, j, o, stop, any_step_zero);
-    RadialOut<double> ro = radial_eval<double, CASE>(rc, lad, tau, DO_PHI_T);
-    if (a.r) a.r[o] = ro.r;
-    bad |= !isfinite(ro.r);
-    if (DO_PHI_T) {
-      const double Gph = a.ph[o];
-      const double Gt = a.t[o];
-      Sample<double> s;
-      combine(cmb, tau, ro, 0.0, Gph, Gt, s);
-      a.t[o] = s.t;
-      a.ph[o] = s.ph;
-      if (a.sig) a.sig[o] = s.sig;
-      bad |= !(isfinite(s.t) && isfinite(s.ph) && isfinite(s.sig));
-    }
-  }
-  return bad;
+  const uint32_t st = theta_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
+  if (a.status && st) atomicOr(&a.status[i], st);
 }
 
 template <bool DO_PHI_T>
@@ -215,23 +116,8 @@ __global__ __launch_bounds__(kBlock) void kg_radial(DevParams p, SampleArgs a) {
   const int64_t G = p.ngeo;
   const int64_t j0 = (G * blockIdx.y) / a.chunks;
   const int64_t j1 = (G * (blockIdx.y + 1)) / a.chunks;
-  RadialConsts<double> rc;
-#pragma unroll
-  for (int s = 0; s < R_NSLOT; ++s) rc.v[s] = a.plan[static_cast<int64_t>(PL_R + s) * p.npix + i];
-  rc.rcase = static_cast<int>(a.plan[static_cast<int64_t>(PL_RCASE) * p.npix + i]);
-  const double lam = a.plan[static_cast<int64_t>(PL_LAM) * p.npix + i];
-  const double stop = a.plan[static_cast<int64_t>(PL_STOP) * p.npix + i];
-  const bool any_step_zero = (a.gflags[0] & GF_ANY_STEP_ZERO) != 0;
-  const CombineConsts<double> cmb = combine_consts(p.c, lam);
-  JacobiLadder<double> lad;
-  jacobi_ladder(rc.v[R_k], lad);
-  bool bad;
-  switch (rc.rcase) {
-    case 4: bad = radial_loop<4, DO_PHI_T>(p, a, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
-    case 3: bad = radial_loop<3, DO_PHI_T>(p, a, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
-    default: bad = radial_loop<2, DO_PHI_T>(p, a, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
-  }
-  if (a.status && (bad || lad.capped)) atomicOr(&a.status[i], (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u));
+  const uint32_t st = radial_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
+  if (a.status && st) atomicOr(&a.status[i], st);
 }
 
 __global__ __launch_bounds__(kBlock) void kg_ellip_pi(const double* __restrict__ n, const double* __restrict__ phi,
@@ -351,7 +237,7 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   if (chunks <= 0) chunks = pick_chunks(d.npix, ngeo);
   if (chunks > ngeo) chunks = static_cast<int>(ngeo);
   dim3 grid(nb, static_cast<unsigned>(chunks));
-  SampleArgs a{plan, gflags, tau_in, tau, t, r, th, ph, sig, status, chunks};
+  SampleArgs a{plan, gflags, SampleOut{tau_in, tau, t, r, th, ph, sig, d.npix, ngeo}, status, chunks};
   if (d.do_phi_t) {
     hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);
     int rc = check_launch("kg_theta");
