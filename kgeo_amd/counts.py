@@ -42,3 +42,39 @@ def n_poloidal_orbits(a, th_o, alpha, beta, tau):
     _, npol = crossings_device(a, th_o, np.atleast_1d(alpha), np.atleast_1d(beta), np.asarray(tau, dtype=float))
     torch.cuda.current_stream().synchronize()
     return npol.cpu().numpy()
+
+
+def ellip_pi_device(n, phi, m):
+    """Pi(n; phi | m) in the A&S convention with the special cases of the
+    reference's ellippi_arr plug-in (scipy_ellip_binding.py:87-237).  n and m
+    must already have phi's shape (the reference broadcasts them with
+    np.outer, :102-106).  Returns a tensor of phi's shape."""
+    lib = _lib.load()
+    dev = _device()
+    ph = _as_device_f64(phi, dev)
+    nn = _as_device_f64(n, dev)
+    mm = _as_device_f64(m, dev)
+    if nn.shape != ph.shape or mm.shape != ph.shape:
+        raise ValueError("n, phi and m must have the same shape")
+    out = torch.empty_like(ph)
+    _lib.check(lib.kgeo_ellip_pi(_ptr(nn), _ptr(ph), _ptr(mm), _ptr(out), ph.numel(), _stream_handle()),
+               "kgeo_ellip_pi")
+    return out
+
+
+def ellippi_arr(n, phi, m):
+    """Drop-in for the reference's ellip_pi_arr(n, phi, m), numpy in and out
+    (scipy_ellip_binding.py:87-237): scalars become 1-element arrays, the
+    three last dimensions must agree (same message as :99-100), and 1-D n or
+    m is repeated along phi's leading axis (:103-106)."""
+    n, phi, m = (np.asarray(x, dtype=np.float64).reshape(-1) if np.ndim(x) == 0 else
+                 np.asarray(x, dtype=np.float64) for x in (n, phi, m))
+    if not (n.shape[-1] == phi.shape[-1] == m.shape[-1]):
+        raise Exception("inputs to ellip_pi_arr do not have same last dimension!")
+    if n.ndim != phi.ndim:
+        n = np.outer(np.ones(phi.shape[0]), n)
+    if m.ndim != phi.ndim:
+        m = np.outer(np.ones(phi.shape[0]), m)
+    out = ellip_pi_device(np.ascontiguousarray(n), np.ascontiguousarray(phi), np.ascontiguousarray(m))
+    torch.cuda.current_stream().synchronize()
+    return out.cpu().numpy()

@@ -292,6 +292,8 @@ template <class T> KG_INLINE T ellip_F_amp(const LegendreBase<T>& L, const T& Kc
 // ---------------------------------------------------------------------------
 template <class T> struct NegAmp {
   T sinphi;  // sin(am(u|k)) unreduced, as the reference's sinPhi_tau
+  T cosphi;  // cn(u|k)
+  T dnk;     // dn(u|k) = 1/dn(x|m')
   Amp<T> A;  // reduced amplitude for the Legendre integrals
 };
 
@@ -300,6 +302,8 @@ KG_INLINE NegAmp<T> neg_amplitude(const T& u, const T& sq, const JacobiLadder<T>
   NegAmp<T> R;
   if (kfabs(val(u)) < kEP) {
     R.sinphi = u;
+    R.cosphi = T(1.0);
+    R.dnk = T(1.0);
     R.A.j = 0.0;
     R.A.s = u;
     R.A.c = T(1.0);
@@ -311,6 +315,8 @@ KG_INLINE NegAmp<T> neg_amplitude(const T& u, const T& sq, const JacobiLadder<T>
   T sphi = sq * sn / dn;
   T cphi = cn / dn;
   R.sinphi = sphi;
+  R.cosphi = cphi;
+  R.dnk = 1.0 / dn;
   R.A.j = krint(val(x) / (2.0 * val(lad.K)));
   const double sg = parity_sign(R.A.j);
   R.A.s = sg * sphi;
@@ -431,36 +437,74 @@ KG_INLINE void theta_consts(const RayScalars<T>& c, const PixelSetup<T>& px, boo
   }
 }
 
+// State of the theta integrals at one Jacobi argument u (= snarg): the
+// Jacobi functions of parameter k at u, and the Legendre integrals at
+// amplitude am(u|k).  The sample loops either evaluate it directly
+// (theta_state) or advance it along the Mino-time grid with the addition
+// theorems (kg_step.hpp); theta_out turns it into theta, G_phi, G_t.
+template <class T> struct ThState {
+  T s, c, d;   // sn, cn, dn (u | k): s is the reference's sinPhi_tau
+  T F, EmF;    // F(am u | k) (= u) and E - F
+  T Pi;        // Pi(n; am u | k)
+};
+
 // KIND = THK_* fixes the motion class at compile time (the kernels' loops);
 // KIND = -1 reads it from tc.kind.
 template <class T, int KIND = -1>
-KG_INLINE void theta_eval(const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, const T& tau, bool do_phi_t, T& th,
-                          T& Gph, T& Gt) {
+KG_INLINE void theta_state(const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, const T& snarg, bool do_phi_t,
+                           ThState<T>& st) {
   const T* v = tc.v;
   const int kind = (KIND >= 0) ? KIND : tc.kind;
-  T snarg = v[TH_w] * (-tau + v[TH_sG]);
   NegAmp<T> R = neg_amplitude(snarg, v[TH_sq], lad);
-  if (kind == THK_VORT) {
-    th = kacos(v[TH_h] * ksqrt(v[TH_A] + v[TH_B] * (R.sinphi * R.sinphi)));
-  } else {
-    th = kacos(v[TH_A] * R.sinphi);
-  }
+  st.s = R.sinphi;
+  st.c = R.cosphi;
+  st.d = R.dnk;
   if (!do_phi_t) return;
   const T& k = v[TH_k];
   LegendreFam<T, 1> fam;
   const T nn[1] = {v[TH_n]};
   const T pc[1] = {v[TH_Pic]};
   legendre_family(R.A, k, v[TH_Kk], v[TH_EmKk], nn, pc, kind != THK_ORD_K0, fam);
-  Gph = v[TH_pref] * fam.Pi[0] - v[TH_sGph];
+  st.F = fam.F;
+  st.EmF = fam.EmF;
+  st.Pi = fam.Pi[0];
+}
+
+template <class T, int KIND = -1>
+KG_INLINE void theta_out(const ThetaConsts<T>& tc, const T& snarg, const ThState<T>& st, bool do_phi_t, T& th, T& Gph,
+                         T& Gt) {
+  const T* v = tc.v;
+  const int kind = (KIND >= 0) ? KIND : tc.kind;
+  if (kind == THK_VORT) {
+    th = kacos(v[TH_h] * ksqrt(v[TH_A] + v[TH_B] * (st.s * st.s)));
+  } else {
+    th = kacos(v[TH_A] * st.s);
+  }
+  if (!do_phi_t) return;
+  const T& k = v[TH_k];
+  Gph = v[TH_pref] * st.Pi - v[TH_sGph];
   if (kind == THK_ORD) {
-    Gt = -(v[TH_B] * fam.EmF) / (2.0 * k) - v[TH_sGt];
+    Gt = -(v[TH_B] * st.EmF) / (2.0 * k) - v[TH_sGt];
   } else if (kind == THK_ORD_K0) {
     // k == 0: m' = 0, am(u|0) = u (kerr_raytracing_ana.py:290-291)
     T Phi = snarg;
     Gt = -v[TH_B] * 0.125 * (-2.0 * Phi + ksin(2.0 * Phi)) - v[TH_sGt];
   } else {
-    Gt = v[TH_C] * (fam.F + fam.EmF) - v[TH_sGt];
+    Gt = v[TH_C] * (st.F + st.EmF) - v[TH_sGt];
   }
+}
+
+template <class T> KG_INLINE T theta_arg(const ThetaConsts<T>& tc, const T& tau) {
+  return tc.v[TH_w] * (-tau + tc.v[TH_sG]);
+}
+
+template <class T, int KIND = -1>
+KG_INLINE void theta_eval(const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, const T& tau, bool do_phi_t, T& th,
+                          T& Gph, T& Gt) {
+  T snarg = theta_arg(tc, tau);
+  ThState<T> st;
+  theta_state<T, KIND>(tc, lad, snarg, do_phi_t, st);
+  theta_out<T, KIND>(tc, snarg, st, do_phi_t, th, Gph, Gt);
 }
 
 // ---------------------------------------------------------------------------
@@ -747,18 +791,45 @@ template <class T> struct RadialOut {
   T r, Ip, Im, I1, I2;
 };
 
-// r(tau) and, with do_phi_t, the building blocks I_+, I_-, I_1, I_2
-// CASE = 2, 3, 4 fixes the radial case at compile time; CASE = -1 reads rc.rcase.
-template <class T, int CASE = -1>
-KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder<T>& lad, const T& tau,
-                                   bool do_phi_t) {
-  RadialOut<T> o;
+// State of the radial integrals at one Jacobi argument X: sn, cn, dn (X | k)
+// and the Legendre integrals at amplitude am(X | k) (F = X, E - F and the
+// case's three third-kind integrals).  radial_state evaluates it directly;
+// kg_step.hpp advances it along the grid; radial_out turns it into r and
+// I_+, I_-, I_1, I_2.
+template <class T> struct RState {
+  T s, c, d;
+  T F, EmF;
+  T Pi[3];
+};
+
+template <class T> KG_INLINE T radial_arg(const RadialConsts<T>& rc, const T& tau) {
+  return rc.v[R_U0] - rc.v[R_Xs] * tau;
+}
+
+// The case's Pi parameters n_i and complete integrals, in Legendre-family order.
+template <class T, int CASE>
+KG_INLINE void radial_pi_params(const RadialConsts<T>& rc, T (&nn)[3], T (&pc)[3]) {
   const T* v = rc.v;
   const int rcase = (CASE >= 0) ? CASE : rc.rcase;
+  if (rcase == 4) {
+    nn[0] = v[C4_n0]; nn[1] = v[C4_np]; nn[2] = v[C4_nm];
+    pc[0] = v[C4_Pc0]; pc[1] = v[C4_Pcp]; pc[2] = v[C4_Pcm];
+  } else if (rcase == 3) {
+    const bool hasm = val(v[C3_hasm]) != 0.0;
+    nn[0] = v[C3_nn0]; nn[1] = v[C3_nnp]; nn[2] = hasm ? v[C3_nnm] : T(0.0);
+    pc[0] = v[C3_Pc0]; pc[1] = v[C3_Pcp]; pc[2] = v[C3_Pcm];
+  } else {
+    nn[0] = v[C2_n1]; nn[1] = v[C2_np]; nn[2] = v[C2_nm];
+    pc[0] = v[C2_Pc1]; pc[1] = v[C2_Pcp]; pc[2] = v[C2_Pcm];
+  }
+}
+
+// dv = Xs * tau (the distance from the observer's argument U0)
+template <class T, int CASE = -1>
+KG_INLINE void radial_state(const RadialConsts<T>& rc, const JacobiLadder<T>& lad, const T& X, const T& dv,
+                            bool do_phi_t, RState<T>& st) {
+  const T* v = rc.v;
   const T& k = v[R_k];
-  T sn, cn, dn;
-  T dv = v[R_Xs] * tau;
-  T X = v[R_U0] - dv;
   if (kfabs(val(dv)) <= 0.25 * val(v[R_K])) {
     // near the observer: addition theorem about u0 (DLMF 22.8.1-3), exact at tau = 0
     T s2, c2, d2;
@@ -767,12 +838,36 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     const T& c1 = v[R_cn0];
     const T& d1 = v[R_dn0];
     T inv = 1.0 / (1.0 - k * (s1 * s1) * (s2 * s2));
-    sn = (s1 * c2 * d2 - c1 * s2 * d1) * inv;
-    cn = (c1 * c2 + s1 * d1 * s2 * d2) * inv;
-    dn = (d1 * d2 + k * s1 * c1 * s2 * c2) * inv;
+    st.s = (s1 * c2 * d2 - c1 * s2 * d1) * inv;
+    st.c = (c1 * c2 + s1 * d1 * s2 * d2) * inv;
+    st.d = (d1 * d2 + k * s1 * c1 * s2 * c2) * inv;
   } else {
-    jacobi_eval(lad, X, sn, cn, dn);
+    jacobi_eval(lad, X, st.s, st.c, st.d);
   }
+  if (!do_phi_t) return;
+  Amp<T> A = amp_from_jacobi(st.s, st.c, val(X), val(v[R_K]));
+  LegendreFam<T, 3> fam;
+  T nn[3], pc[3];
+  radial_pi_params<T, CASE>(rc, nn, pc);
+  legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
+  st.F = fam.F;
+  st.EmF = fam.EmF;
+  st.Pi[0] = fam.Pi[0];
+  st.Pi[1] = fam.Pi[1];
+  st.Pi[2] = fam.Pi[2];
+}
+
+// r(tau) and, with do_phi_t, the building blocks I_+, I_-, I_1, I_2
+// CASE = 2, 3, 4 fixes the radial case at compile time; CASE = -1 reads rc.rcase.
+template <class T, int CASE = -1>
+KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const RState<T>& st, bool do_phi_t) {
+  RadialOut<T> o;
+  const T* v = rc.v;
+  const int rcase = (CASE >= 0) ? CASE : rc.rcase;
+  const T& k = v[R_k];
+  const T& sn = st.s;
+  const T& cn = st.c;
+  const T& dn = st.d;
   if (rcase == 4) {
     const T& a2 = v[C4_a2];
     const T& b1 = v[C4_b1];
@@ -780,17 +875,12 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     // -a2 (g0 - sc)/(1 + g0 sc) - b1 with sc = sn/cn, written without 1/cn
     o.r = -a2 * (g0 * cn - sn) / (cn + g0 * sn) - b1;
     if (!do_phi_t) return o;
-    Amp<T> A = amp_from_jacobi(sn, cn, val(X), val(v[R_K]));
-    LegendreFam<T, 3> fam;
-    const T nn[3] = {v[C4_n0], v[C4_np], v[C4_nm]};
-    const T pc[3] = {v[C4_Pc0], v[C4_Pcp], v[C4_Pcm]};
-    legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
-    const T& F = fam.F;
-    T E = F + fam.EmF;
-    T S1a0 = S1_block(g0, fam.Pi[0], F, dn, k);
+    const T& F = st.F;
+    T E = F + st.EmF;
+    T S1a0 = S1_block(g0, st.Pi[0], F, dn, k);
     T S2a0 = S2_block(g0, S1a0, F, E, sn, cn, dn, k);
-    T S1ap = S1_block(v[C4_gp], fam.Pi[1], F, dn, k);
-    T S1am = S1_block(v[C4_gm], fam.Pi[2], F, dn, k);
+    T S1ap = S1_block(v[C4_gp], st.Pi[1], F, dn, k);
+    T S1am = S1_block(v[C4_gm], st.Pi[2], F, dn, k);
     T Pi1 = v[C4_P1f] * (S1a0 - v[C4_S1b0]);
     T Pi2 = v[C4_P2f] * (S2a0 - v[C4_S2b0]);
     T Pip = v[C4_Ppf] * (S1ap - v[C4_S1bp]);
@@ -809,18 +899,13 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     T den = (BB - AA) + (BB + AA) * cn;
     o.r = num / den;
     if (!do_phi_t) return o;
-    Amp<T> A = amp_from_jacobi(sn, cn, val(X), val(v[R_K]));
     const bool hasm = val(v[C3_hasm]) != 0.0;
-    LegendreFam<T, 3> fam;
-    const T nn[3] = {v[C3_nn0], v[C3_nnp], hasm ? v[C3_nnm] : T(0.0)};
-    const T pc[3] = {v[C3_Pc0], v[C3_Pcp], v[C3_Pcm]};
-    legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
-    const T& F = fam.F;
-    T E = F + fam.EmF;
-    T R1a0 = R1_block(v[C3_al0], fam.Pi[0], sn, dn, k);
+    const T& F = st.F;
+    T E = F + st.EmF;
+    T R1a0 = R1_block(v[C3_al0], st.Pi[0], sn, dn, k);
     T R2a0 = R2_block(v[C3_al0], v[C3_nn0], R1a0, F, E, sn, cn, dn, k);
-    T R1ap = R1_block(v[C3_alp], fam.Pi[1], sn, dn, k);
-    T R1am = hasm ? R1_block(v[C3_alm], fam.Pi[2], sn, dn, k) : T(0.0);
+    T R1ap = R1_block(v[C3_alp], st.Pi[1], sn, dn, k);
+    T R1am = hasm ? R1_block(v[C3_alm], st.Pi[2], sn, dn, k) : T(0.0);
     T Pi1 = v[C3_c0] * (R1a0 - v[C3_R1b0]);
     T Pi2 = (v[C3_c0] * v[C3_c0]) * (R2a0 - v[C3_R2b0]);
     T Pip = v[C3_cp] * (R1ap - v[C3_R1bp]);
@@ -843,22 +928,27 @@ KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder
     T dsn2 = 2.0 * sn * cn * dn * v[C2_dX];
     T drs = -(rr31 * rr43 * rr41 * dsn2 / (den * den));
     T H = drs / (o.r - rr3) - v[C2_drso] / v[C2_ro3];
-    Amp<T> A = amp_from_jacobi(sn, cn, val(X), val(v[R_K]));
-    LegendreFam<T, 3> fam;
-    const T nn[3] = {v[C2_n1], v[C2_np], v[C2_nm]};
-    const T pc[3] = {v[C2_Pc1], v[C2_Pcp], v[C2_Pcm]};
-    legendre_family(A, k, v[R_K], v[R_EmK], nn, pc, true, fam);
-    T Ea = fam.F + fam.EmF;
+    T Ea = st.F + st.EmF;
     T E = v[C2_sq] * (Ea - v[C2_Eb]);
-    T Pi1 = v[C2_c] * (fam.Pi[0] - v[C2_Pb1]);
-    T Pip = v[C2_fp] * (fam.Pi[1] - v[C2_Pbp]);
-    T Pim = v[C2_fm] * (fam.Pi[2] - v[C2_Pbm]);
+    T Pi1 = v[C2_c] * (st.Pi[0] - v[C2_Pb1]);
+    T Pip = v[C2_fp] * (st.Pi[1] - v[C2_Pbp]);
+    T Pim = v[C2_fm] * (st.Pi[2] - v[C2_Pbm]);
     o.I1 = rr3 * (-tau) + rr43 * Pi1;
     o.I2 = H - v[C2_h12] * (-tau) - E;
     o.Ip = tau / v[C2_rrp3] - Pip;
     o.Im = tau / v[C2_rrm3] - Pim;
   }
   return o;
+}
+
+template <class T, int CASE = -1>
+KG_INLINE RadialOut<T> radial_eval(const RadialConsts<T>& rc, const JacobiLadder<T>& lad, const T& tau,
+                                   bool do_phi_t) {
+  const T dv = rc.v[R_Xs] * tau;
+  const T X = rc.v[R_U0] - dv;
+  RState<T> st;
+  radial_state<T, CASE>(rc, lad, X, dv, do_phi_t, st);
+  return radial_out<T, CASE>(rc, tau, st, do_phi_t);
 }
 
 // ---------------------------------------------------------------------------

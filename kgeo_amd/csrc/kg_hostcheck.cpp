@@ -15,6 +15,7 @@
 #include "kg_dual.hpp"
 #include "kg_equatorial.hpp"
 #include "kg_geodesic.hpp"
+#include "kg_sample.hpp"
 
 using namespace kg;
 
@@ -100,42 +101,59 @@ void kgh_setup(const double* sc, const double* alpha, const double* beta, int64_
   }
 }
 
-// Full raytrace of the kernels' algorithm on the CPU: outputs [ngeo][npix].
-// tau_in (optional, [ngeo][npix]) replaces the linspace grid.
+// Full raytrace of the kernels' algorithm on the CPU: the same plan layout
+// ([PL_N][npix] slot-major), image-wide flags and per-pixel sample loops
+// (kg_sample.hpp: theta_pixel then radial_pixel) as kg_setup / kg_theta /
+// kg_radial, with ngeo split into `chunks` contiguous ranges per pixel as the
+// kernels' grid does.  Outputs [ngeo][npix]; tau_in (optional, [ngeo][npix])
+// replaces the linspace grid, which also makes every sample a direct
+// evaluation.  status (optional) receives the kernels' per-pixel bits.
+void kgh_raytrace_loops(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo,
+                        int do_phi_t, const double* tau_in, int chunks, double* tau, double* t, double* r, double* th,
+                        double* ph, double* sig, int32_t* nmax_eq, uint32_t* status) {
+  RayScalars<double> c = scalars_from(sc);
+  std::vector<double> plan(static_cast<size_t>(npix) * PL_N);
+  std::vector<int8_t> rcl(npix), tcl(npix);
+  std::vector<double> tt(npix);
+  std::vector<uint32_t> st(npix, 0u);
+  uint32_t gflags = 0;
+  for (int64_t i = 0; i < npix; ++i) {
+    SetupOut so = setup_pixel(c, ngeo, do_phi_t != 0, alpha[i], beta[i], plan.data(), npix, i, rcl.data(),
+                              tcl.data(), nmax_eq, tt.data(), st.data(), i);
+    if (so.eta_pos) gflags |= GF_ANY_ETA_POS;
+    if (so.step_zero) gflags |= GF_ANY_STEP_ZERO;
+  }
+  const int64_t G = ngeo;
+  if (chunks < 1) chunks = 1;
+  SampleOut o{tau_in, tau, t, r, th, ph, sig, npix, G};
+  std::vector<double> tbuf, pbuf;
+  if (!do_phi_t) {
+    o.t = nullptr;
+    o.ph = nullptr;
+  }
+  for (int64_t i = 0; i < npix; ++i) {
+    for (int ch = 0; ch < chunks; ++ch) {
+      const int64_t j0 = (G * ch) / chunks, j1 = (G * (ch + 1)) / chunks;
+      uint32_t bits;
+      if (do_phi_t) {
+        bits = theta_pixel<true>(plan.data(), npix, i, gflags, o, i, j0, j1);
+        bits |= radial_pixel<true>(c, plan.data(), npix, i, gflags, o, i, j0, j1);
+      } else {
+        bits = theta_pixel<false>(plan.data(), npix, i, gflags, o, i, j0, j1);
+        bits |= radial_pixel<false>(c, plan.data(), npix, i, gflags, o, i, j0, j1);
+      }
+      st[i] |= bits;
+    }
+  }
+  if (status)
+    for (int64_t i = 0; i < npix; ++i) status[i] = st[i];
+}
+
+// Backwards-compatible entry: all samples of a pixel in one range.
 void kgh_raytrace(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo, int do_phi_t,
                   const double* tau_in, double* tau, double* t, double* r, double* th, double* ph, double* sig,
                   int32_t* nmax_eq) {
-  RayScalars<double> c = scalars_from(sc);
-  std::vector<double> plan(static_cast<size_t>(npix) * PL_N);
-  std::vector<double> stop(npix);
-  bool any_eta_pos = false, any_step_zero = false;
-  for (int64_t i = 0; i < npix; ++i) {
-    PixelSetup<double> px = plan_pixel(c, alpha[i], beta[i], do_phi_t != 0, &plan[i * PL_N]);
-    if (val(px.eta) > 0.0) any_eta_pos = true;
-    double st = plan[i * PL_N + PL_STOP];
-    if (ngeo > 1 && st / static_cast<double>(ngeo - 1) == 0.0) any_step_zero = true;
-    stop[i] = st;
-  }
-  for (int64_t i = 0; i < npix; ++i) {
-    PixelConsts<double> pc;
-    plan_unpack(c, &plan[i * PL_N], pc);
-    bool theta_on = !(pc.th.eta_zero && !any_eta_pos);
-    for (int64_t j = 0; j < ngeo; ++j) {
-      double tj = tau_in ? tau_in[j * npix + i] : mino_tau(j, ngeo, stop[i], any_step_zero);
-      Sample<double> s = eval_sample(pc, tj, do_phi_t != 0, theta_on);
-      int64_t o = j * npix + i;
-      if (tau) tau[o] = tj;
-      if (t) t[o] = s.t;
-      if (r) r[o] = s.r;
-      if (th) th[o] = s.th;
-      if (ph) ph[o] = s.ph;
-      if (sig) sig[o] = s.sig;
-    }
-    if (nmax_eq) {
-      double tl = mino_tau(ngeo - 1, ngeo, stop[i], any_step_zero);
-      nmax_eq[i] = nmax_equatorial_at(c, alpha[i], beta[i], tl);
-    }
-  }
+  kgh_raytrace_loops(sc, alpha, beta, npix, ngeo, do_phi_t, tau_in, 1, tau, t, r, th, ph, sig, nmax_eq, nullptr);
 }
 
 // Equatorial image (kg_equatorial.hpp) on the CPU: per-subring arrays are
@@ -174,6 +192,25 @@ void kgh_equatorial_jac(double a, double th_o, double r_o, double p, const doubl
     image[i] = e.image.v;
     for (int k = 0; k < 3; ++k) jac[k * n + i] = e.image.d[k];
   }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// One addition-theorem step of kg_step.hpp for a single third-kind
+// integral: in = {sn, cn, dn, (E-F)/m, Pi} at u and at v; out = the same at
+// u + v.  Returns 1 when the step is taken, 0 when it is refused.
+int kgh_walk_step(const double* at_u, const double* at_v, double m, double n, double* out) {
+  WalkState<1> a, r;
+  WalkStep<1> w;
+  a.s = at_u[0]; a.c = at_u[1]; a.d = at_u[2]; a.D = at_u[3]; a.P[0] = at_u[4];
+  w.b.s = at_v[0]; w.b.c = at_v[1]; w.b.d = at_v[2]; w.b.D = at_v[3]; w.b.P[0] = at_v[4];
+  const double nn[1] = {n};
+  walk_init(w, m, nn);
+  const bool ok = walk_step<1, true>(a, w, r);
+  out[0] = r.s; out[1] = r.c; out[2] = r.d; out[3] = r.D; out[4] = r.P[0];
+  return ok ? 1 : 0;
 }
 
 }  // extern "C"

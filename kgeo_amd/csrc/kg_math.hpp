@@ -58,6 +58,7 @@ KG_INLINE double kfabs(double x) { return ::fabs(x); }
 KG_INLINE double kpow(double x, double y) { return ::pow(x, y); }
 KG_INLINE double kfloor(double x) { return ::floor(x); }
 KG_INLINE double krint(double x) { return ::rint(x); }
+KG_INLINE bool kfinite(double x) { return kfabs(x) < INFINITY; }  // false for NaN
 KG_INLINE double kmax(double a, double b) { return a > b ? a : b; }
 KG_INLINE double parity_sign(double j) { return (::fmod(j, 2.0) == 0.0) ? 1.0 : -1.0; }
 

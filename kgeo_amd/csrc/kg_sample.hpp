@@ -10,10 +10,18 @@
 // Each runs one loop specialised at compile time for the pixel's motion class
 // or radial case, so a wave only keeps its own case's constants live.  The
 // plan is slot-major: slot s of pixel p is plan[s * stride + p].
+//
+// On the linspace grid the loops walk: each sample's state follows from the
+// previous one by the addition theorems (kg_step.hpp), with a direct
+// evaluation at the chunk's first sample, every kAnchorEvery samples, at the
+// last grid sample (tau = stop exactly) and wherever a step is not well
+// conditioned.  With caller-given Mino times (tau_in) every sample is direct.
+// Host-compilable (KG_INLINE), so the CPU test build runs the same loops.
 
 #pragma once
 
 #include "kg_geodesic.hpp"
+#include "kg_step.hpp"
 
 namespace kg {
 
@@ -31,12 +39,12 @@ struct SampleOut {
   int64_t ngeo;
 };
 
-__device__ inline double sample_tau(const SampleOut& o, int64_t j, int64_t idx, double stop, bool any_step_zero) {
+KG_INLINE double sample_tau(const SampleOut& o, int64_t j, int64_t idx, double stop, bool any_step_zero) {
   return o.tau_in ? o.tau_in[idx] : mino_tau(j, o.ngeo, stop, any_step_zero);
 }
 
 template <int KIND, bool DO_PHI_T>
-__device__ inline bool theta_loop(const SampleOut& o, const ThetaConsts<double>& tc, const JacobiLadder<double>& lad,
+KG_INLINE bool theta_loop(const SampleOut& o, const ThetaConsts<double>& tc, const JacobiLadder<double>& lad,
                                   int64_t i, int64_t j0, int64_t j1, double stop, bool any_step_zero) {
   bool bad = false;
   for (int64_t j = j0; j < j1; ++j) {
@@ -46,7 +54,7 @@ __device__ inline bool theta_loop(const SampleOut& o, const ThetaConsts<double>&
     if (KIND != THK_NONE) theta_eval<double, KIND>(tc, lad, tau, DO_PHI_T, th, Gph, Gt);
     if (o.tau_out) o.tau_out[idx] = tau;
     if (o.th) o.th[idx] = th;
-    bad |= !isfinite(th);
+    bad |= !kfinite(th);
     if (DO_PHI_T) {
       o.ph[idx] = Gph;
       o.t[idx] = Gt;
@@ -56,7 +64,7 @@ __device__ inline bool theta_loop(const SampleOut& o, const ThetaConsts<double>&
 }
 
 template <int CASE, bool DO_PHI_T>
-__device__ inline bool radial_loop(const SampleOut& o, const RadialConsts<double>& rc, const JacobiLadder<double>& lad,
+KG_INLINE bool radial_loop(const SampleOut& o, const RadialConsts<double>& rc, const JacobiLadder<double>& lad,
                                    const CombineConsts<double>& cmb, int64_t i, int64_t j0, int64_t j1, double stop,
                                    bool any_step_zero) {
   bool bad = false;
@@ -65,7 +73,7 @@ __device__ inline bool radial_loop(const SampleOut& o, const RadialConsts<double
     const double tau = sample_tau(o, j, idx, stop, any_step_zero);
     RadialOut<double> ro = radial_eval<double, CASE>(rc, lad, tau, DO_PHI_T);
     if (o.r) o.r[idx] = ro.r;
-    bad |= !isfinite(ro.r);
+    bad |= !kfinite(ro.r);
     if (DO_PHI_T) {
       const double Gph = o.ph[idx];
       const double Gt = o.t[idx];
@@ -74,19 +82,175 @@ __device__ inline bool radial_loop(const SampleOut& o, const RadialConsts<double
       o.t[idx] = s.t;
       o.ph[idx] = s.ph;
       if (o.sig) o.sig[idx] = s.sig;
-      bad |= !(isfinite(s.t) && isfinite(s.ph) && isfinite(s.sig));
+      bad |= !(kfinite(s.t) && kfinite(s.ph) && kfinite(s.sig));
     }
   }
   return bad;
 }
 
-__device__ inline uint32_t loop_status(bool bad, const JacobiLadder<double>& lad) {
+// Writers shared by the direct and the walking loops.
+template <bool DO_PHI_T>
+KG_INLINE bool theta_store(const SampleOut& o, int64_t idx, double tau, double th, double Gph, double Gt) {
+  if (o.tau_out) o.tau_out[idx] = tau;
+  if (o.th) o.th[idx] = th;
+  if (DO_PHI_T) {
+    o.ph[idx] = Gph;
+    o.t[idx] = Gt;
+  }
+  return !kfinite(th);
+}
+
+template <bool DO_PHI_T>
+KG_INLINE bool radial_store(const SampleOut& o, const CombineConsts<double>& cmb, int64_t idx, double tau,
+                            const RadialOut<double>& ro) {
+  if (o.r) o.r[idx] = ro.r;
+  bool bad = !kfinite(ro.r);
+  if (DO_PHI_T) {
+    const double Gph = o.ph[idx];
+    const double Gt = o.t[idx];
+    Sample<double> s;
+    combine(cmb, tau, ro, 0.0, Gph, Gt, s);
+    o.t[idx] = s.t;
+    o.ph[idx] = s.ph;
+    if (o.sig) o.sig[idx] = s.sig;
+    bad |= !(kfinite(s.t) && kfinite(s.ph) && kfinite(s.sig));
+  }
+  return bad;
+}
+
+// Theta samples j0..j1-1 on the linspace grid (ordinary or vortical motion),
+// walking u_j = w (s_G - j h) with the addition theorems.
+template <int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<double>& tc, const JacobiLadder<double>& lad,
+                               int64_t i, int64_t j0, int64_t j1, double stop) {
+  const double m = tc.v[TH_k];
+  const double h = stop / static_cast<double>(o.ngeo - 1);
+  WalkStep<1> w;
+  {
+    ThState<double> sb;
+    theta_state<double, KIND>(tc, lad, -tc.v[TH_w] * h, DO_PHI_T, sb);
+    w.b.s = sb.s;
+    w.b.c = sb.c;
+    w.b.d = sb.d;
+    w.b.D = DO_PHI_T ? sb.EmF / m : 0.0;
+    w.b.P[0] = DO_PHI_T ? sb.Pi : 0.0;
+    const double n[1] = {tc.v[TH_n]};
+    walk_init(w, m, n);
+  }
+  WalkState<1> st;
+  bool bad = false;
+  int since = 0;
+  for (int64_t j = j0; j < j1; ++j, ++since) {
+    const int64_t idx = j * o.npix + i;
+    const double tau = mino_tau(j, o.ngeo, stop, false);
+    const double snarg = theta_arg(tc, tau);
+    ThState<double> ts;
+    bool direct = (j == j0) || (since == kAnchorEvery) || (j == o.ngeo - 1);
+    if (!direct) {
+      WalkState<1> nx;
+      if (walk_step<1, DO_PHI_T>(st, w, nx)) {
+        st = nx;
+        ts.s = st.s;
+        ts.c = st.c;
+        ts.d = st.d;
+        ts.F = snarg;
+        ts.EmF = m * st.D;
+        ts.Pi = st.P[0];
+      } else {
+        direct = true;
+      }
+    }
+    if (direct) {
+      theta_state<double, KIND>(tc, lad, snarg, DO_PHI_T, ts);
+      st.s = ts.s;
+      st.c = ts.c;
+      st.d = ts.d;
+      st.D = DO_PHI_T ? ts.EmF / m : 0.0;
+      st.P[0] = DO_PHI_T ? ts.Pi : 0.0;
+      since = 0;
+    }
+    double th = 0.0, Gph = 0.0, Gt = 0.0;
+    theta_out<double, KIND>(tc, snarg, ts, DO_PHI_T, th, Gph, Gt);
+    bad |= theta_store<DO_PHI_T>(o, idx, tau, th, Gph, Gt);
+  }
+  return bad;
+}
+
+// Radial samples j0..j1-1 on the linspace grid, walking X_j = U0 - Xs j h.
+template <int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<double>& rc, const JacobiLadder<double>& lad,
+                                const CombineConsts<double>& cmb, int64_t i, int64_t j0, int64_t j1, double stop) {
+  const double m = rc.v[R_k];
+  const double h = stop / static_cast<double>(o.ngeo - 1);
+  WalkStep<3> w;
+  {
+    const double dX = -rc.v[R_Xs] * h;
+    RState<double> sb;
+    radial_state<double, CASE>(rc, lad, dX, INFINITY, DO_PHI_T, sb);  // plain Jacobi at dX
+    w.b.s = sb.s;
+    w.b.c = sb.c;
+    w.b.d = sb.d;
+    w.b.D = DO_PHI_T ? sb.EmF / m : 0.0;
+    double n[3], pc[3];
+    radial_pi_params<double, CASE>(rc, n, pc);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) w.b.P[q] = DO_PHI_T ? sb.Pi[q] : 0.0;
+    walk_init(w, m, n);
+  }
+  WalkState<3> st;
+  bool bad = false;
+  int since = 0;
+  for (int64_t j = j0; j < j1; ++j, ++since) {
+    const int64_t idx = j * o.npix + i;
+    const double tau = mino_tau(j, o.ngeo, stop, false);
+    const double dv = rc.v[R_Xs] * tau;
+    const double X = rc.v[R_U0] - dv;
+    RState<double> rs;
+    bool direct = (j == j0) || (since == kAnchorEvery) || (j == o.ngeo - 1);
+    if (!direct) {
+      WalkState<3> nx;
+      if (walk_step<3, DO_PHI_T>(st, w, nx)) {
+        st = nx;
+        rs.s = st.s;
+        rs.c = st.c;
+        rs.d = st.d;
+        rs.F = X;
+        rs.EmF = m * st.D;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) rs.Pi[q] = st.P[q];
+      } else {
+        direct = true;
+      }
+    }
+    if (direct) {
+      radial_state<double, CASE>(rc, lad, X, dv, DO_PHI_T, rs);
+      st.s = rs.s;
+      st.c = rs.c;
+      st.d = rs.d;
+      st.D = DO_PHI_T ? rs.EmF / m : 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) st.P[q] = DO_PHI_T ? rs.Pi[q] : 0.0;
+      since = 0;
+    }
+    RadialOut<double> ro = radial_out<double, CASE>(rc, tau, rs, DO_PHI_T);
+    bad |= radial_store<DO_PHI_T>(o, cmb, idx, tau, ro);
+  }
+  return bad;
+}
+
+// Walking needs the linspace grid with at least one interior step: not for
+// caller-given Mino times and not for numpy's zero-step branch.
+KG_INLINE bool walk_ok(const SampleOut& o, bool any_step_zero) {
+  return o.tau_in == nullptr && !any_step_zero && o.ngeo > 2;
+}
+
+KG_INLINE uint32_t loop_status(bool bad, const JacobiLadder<double>& lad) {
   return (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u);
 }
 
 // Theta part of samples j0..j1-1 of output pixel i, whose plan column is p.
 template <bool DO_PHI_T>
-__device__ inline uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+KG_INLINE uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
                                        const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
   ThetaConsts<double> tc;
 #pragma unroll
@@ -100,10 +264,17 @@ __device__ inline uint32_t theta_pixel(const double* __restrict__ plan, int64_t 
   const int k = (eta0 && !any_eta_pos) ? THK_NONE : kind;
   JacobiLadder<double> lad;
   jacobi_ladder(tc.v[TH_mp], lad);
+  const bool walk = walk_ok(o, any_step_zero);
   bool bad;
   switch (k) {
-    case THK_ORD: bad = theta_loop<THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
-    case THK_VORT: bad = theta_loop<THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    case THK_ORD:
+      bad = walk ? theta_loop_walk<THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop)
+                 : theta_loop<THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero);
+      break;
+    case THK_VORT:
+      bad = walk ? theta_loop_walk<THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop)
+                 : theta_loop<THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero);
+      break;
     case THK_ORD_K0: bad = theta_loop<THK_ORD_K0, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
     default: bad = theta_loop<THK_NONE, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
   }
@@ -112,7 +283,7 @@ __device__ inline uint32_t theta_pixel(const double* __restrict__ plan, int64_t 
 
 // Radial part (and the phi, t, sigma combination) of the same samples.
 template <bool DO_PHI_T>
-__device__ inline uint32_t radial_pixel(const RayScalars<double>& c, const double* __restrict__ plan, int64_t stride,
+KG_INLINE uint32_t radial_pixel(const RayScalars<double>& c, const double* __restrict__ plan, int64_t stride,
                                         int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j0,
                                         int64_t j1) {
   RadialConsts<double> rc;
@@ -125,11 +296,21 @@ __device__ inline uint32_t radial_pixel(const RayScalars<double>& c, const doubl
   const CombineConsts<double> cmb = combine_consts(c, lam);
   JacobiLadder<double> lad;
   jacobi_ladder(rc.v[R_k], lad);
+  const bool walk = walk_ok(o, any_step_zero) && lad.n >= 0;
   bool bad;
   switch (rc.rcase) {
-    case 4: bad = radial_loop<4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
-    case 3: bad = radial_loop<3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
-    default: bad = radial_loop<2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero); break;
+    case 4:
+      bad = walk ? radial_loop_walk<4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
+                 : radial_loop<4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
+      break;
+    case 3:
+      bad = walk ? radial_loop_walk<3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
+                 : radial_loop<3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
+      break;
+    default:
+      bad = walk ? radial_loop_walk<2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
+                 : radial_loop<2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
+      break;
   }
   return loop_status(bad, lad);
 }
@@ -141,7 +322,7 @@ struct SetupOut {
   bool step_zero;
 };
 
-__device__ inline SetupOut setup_pixel(const RayScalars<double>& c, int64_t ngeo, bool do_phi_t, double al, double be,
+KG_INLINE SetupOut setup_pixel(const RayScalars<double>& c, int64_t ngeo, bool do_phi_t, double al, double be,
                                        double* __restrict__ plan, int64_t stride, int64_t p, int8_t* rclass,
                                        int8_t* thclass, int32_t* nmax_eq, double* tau_tot, uint32_t* status,
                                        int64_t i) {
@@ -165,9 +346,11 @@ __device__ inline SetupOut setup_pixel(const RayScalars<double>& c, int64_t ngeo
   return so;
 }
 
+#if defined(__HIPCC__)
 __device__ inline void wave_or(uint32_t* word, bool pred, uint32_t bit) {
   const unsigned long long b = __ballot(pred ? 1 : 0);
   if (b != 0ull && __lane_id() == 0) atomicOr(word, bit);
 }
+#endif
 
 }  // namespace kg

@@ -123,13 +123,23 @@ __global__ __launch_bounds__(kBlock) void kg_batch_setup(BatchDev d) {
   wave_or(&d.flags[im], so.step_zero, GF_ANY_STEP_ZERO);
 }
 
+// The work-item fetch is written so that the compiler sees it as uniform:
+// every lane issues the atomic (lane 0 adds 1, the others 0; the backend
+// combines them into one atomic per wave) and the value is taken with
+// readfirstlane, so `item` lives in a scalar register and the exit test is
+// a scalar branch.  A lane-0-only atomic followed by a lane shuffle lets the
+// structurizer split the wave's lanes into two loop nests, in which the
+// shuffle re-reads an inactive lane 0 and the loop never ends.
+__device__ inline uint32_t fetch_item(uint32_t* counter, int lane) {
+  const uint32_t got = atomicAdd(counter, lane == 0 ? 1u : 0u);
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
+}
+
 __global__ __launch_bounds__(kBlock) void kg_batch_persistent(BatchDev d) {
   const int lane = __lane_id();
   for (;;) {
-    uint32_t item = 0u;
-    if (lane == 0) item = atomicAdd(d.counter, 1u);
-    item = __shfl(item, 0);
-    if (item >= d.total_items) break;  // wave-uniform; every wave gets here once the queue is empty
+    const uint32_t item = fetch_item(d.counter, lane);
+    if (item >= d.total_items) break;  // scalar branch: the whole wave leaves once the queue is empty
     const int im = find_image(d.imgs, d.nimg, static_cast<int64_t>(item), true);
     const BatchImage& B = d.imgs[im];
     const int64_t li = static_cast<int64_t>(item) - B.item_off;
@@ -154,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void kg_batch_persistent(BatchDev d) {
       }
       if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
     }
-    if (lane == 0) atomicAdd(&d.done[im], 1u);
+    atomicAdd(&d.done[im], lane == 0 ? 1u : 0u);
   }
 }
 

@@ -190,3 +190,24 @@ def test_ellip_pi_plugin():
     assert np.max(np.abs(P - g["Pi"]) / np.maximum(np.abs(g["Pi"]), 1e-3)) < 1e-12
     want = O.ellip_pi(g["n"], g["phi"], g["m"])
     assert np.max(np.abs(P - want) / np.maximum(np.abs(want), 1e-3)) < 1e-12
+
+
+def test_ellippi_arr_dropin_broadcasting():
+    """The numpy-level plug-in: 1-D n, m repeated along phi's leading axis,
+    scalars accepted, mismatched last dimensions rejected with the
+    reference's message (scipy_ellip_binding.py:94-106)."""
+    from kgeo_amd.counts import ellippi_arr
+    rng = np.random.default_rng(5)
+    phi = rng.uniform(-4.0, 4.0, (7, 11))
+    n = rng.uniform(-2.0, 0.9, 11)
+    m = rng.uniform(-3.0, 0.95, 11)
+    got = ellippi_arr(n, phi, m)
+    want = O.ellip_pi(np.outer(np.ones(7), n), phi, np.outer(np.ones(7), m))
+    assert got.shape == phi.shape
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = np.isfinite(want)
+    assert np.max(np.abs(got[ok] - want[ok]) / np.maximum(np.abs(want[ok]), 1e-3)) < 1e-12
+    s = ellippi_arr(0.3, 0.7, 0.5)
+    assert s.shape == (1,) and abs(s[0] - O.ellip_pi(0.3, 0.7, 0.5)[0]) < 1e-14
+    with pytest.raises(Exception, match="same last dimension"):
+        ellippi_arr(np.zeros(3), np.zeros(4), np.zeros(4))

@@ -1,0 +1,109 @@
+"""The sample loops' walk along the Mino-time grid (kgeo_amd/csrc/kg_step.hpp).
+
+1. One step of the addition theorems (Jacobi sn/cn/dn, E, and the third-kind
+   integral Pi, including the principal value for n > 1) against values
+   computed independently: mpmath's Jacobi functions, scipy's E(phi|m) and the
+   oracle's Pi (itself pinned to mpmath in test_oracle_elliptic.py).  A step
+   that would cross a pole of the Pi integrand must be refused.
+2. Whole rays: the walking loops (CPU build of the kernels' source) against
+   the same loops forced to evaluate every sample directly (caller-given
+   Mino times), over several spins, inclinations, ngeo and chunkings.
+"""
+
+import ctypes
+
+import mpmath as mp
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from oracle import kgeo_oracle as O
+
+from .kgeo_testlib import HOSTCHECK, Host, errors, grid
+
+
+def _lib():
+    return ctypes.CDLL(HOSTCHECK)
+
+
+def _truth(u, m, n):
+    """(sn, cn, dn, (E - F)/m, Pi) at argument u, |u| < K(m), for parameter m."""
+    mp.mp.dps = 30
+    s, c, d = (float(mp.re(mp.ellipfun(f, u, m=m))) for f in ("sn", "cn", "dn"))
+    phi = float(np.arcsin(s))  # |u| < K: am(u) = asin(sn u)
+    E = float(sp.ellipeinc(phi, m))
+    P = float(O.ellip_pi(np.array([n]), np.array([phi]), np.array([m]))[0])
+    return np.array([s, c, d, (E - u) / m, P])
+
+
+def _step(lib, at_u, at_v, m, n):
+    out = np.zeros(5)
+    P = lambda x: np.ascontiguousarray(x, dtype=float).ctypes.data_as(ctypes.c_void_p)
+    ok = lib.kgh_walk_step(P(at_u), P(at_v), ctypes.c_double(m), ctypes.c_double(n), P(out))
+    return ok, out
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_one_step_against_independent_values(seed):
+    lib = _lib()
+    lib.kgh_walk_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_void_p]
+    rng = np.random.default_rng(seed)
+    taken = 0
+    for _ in range(40):
+        m = float(rng.choice([rng.uniform(-2.5, -0.05), rng.uniform(0.05, 0.97)]))
+        n = float(rng.choice([rng.uniform(-3.0, 0.9), rng.uniform(1.2, 3.0)]))
+        K = float(sp.ellipk(m)) if m > 0 else float(mp.ellipk(m))
+        u = float(rng.uniform(-0.8, 0.8) * K)
+        v = float(rng.uniform(-0.02, 0.02))
+        want = _truth(u + v, m, n)
+        ok, got = _step(lib, _truth(u, m, n), _truth(v, m, n), m, n)
+        if not ok:
+            # refused steps must be the ill-conditioned ones: a pole of the
+            # integrand at or next to the step (n > 1 only)
+            assert n > 1.0
+            continue
+        taken += 1
+        scale = np.maximum(np.abs(want), 1.0)
+        assert np.all(np.abs(got - want) / scale < 2e-14), (m, n, u, v, got, want)
+    assert taken >= 25
+
+
+def test_step_across_a_pole_is_refused():
+    lib = _lib()
+    lib.kgh_walk_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_void_p]
+    m, n = 0.3, 1.6
+    # sn^2(u_p) = 1/n: the principal-value pole
+    phi_p = np.arcsin(np.sqrt(1.0 / n))
+    u_p = float(sp.ellipkinc(phi_p, m))
+    for (u, v) in ((u_p - 0.004, 0.008), (u_p - 0.001, 0.0015), (u_p + 0.003, -0.006)):
+        ok, _ = _step(lib, _truth(u, m, n), _truth(v, m, n), m, n)
+        assert not ok, (u, v)
+
+
+CASES = [
+    (0.94, 17.0, 8.0, 48, 500, 1),
+    (0.94, 17.0, 8.0, 48, 500, 3),
+    (-0.6, 70.0, 7.0, 30, 37, 1),
+    (0.99, 88.0, 8.0, 30, 130, 2),
+    (0.5, 140.0, 6.0, 24, 50, 1),
+    (5.0e-5, 45.0, 8.0, 24, 80, 1),
+    (-0.9, 80.0, 10.0, 40, 400, 7),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"a{c[0]}_th{c[1]}_G{c[4]}_ch{c[5]}")
+def test_walking_loops_match_direct_evaluation(case):
+    a, thd, w, n, G, chunks = case
+    A, B = grid(w, n)
+    A, B = np.ascontiguousarray(A), np.ascontiguousarray(B)
+    th = np.deg2rad(thd)
+    host = Host()
+    walk = host.raytrace_loops(a, 1.0e4, th, A, B, G, chunks=chunks)
+    direct = host.raytrace_loops(a, 1.0e4, th, A, B, G, tau_in=walk["tau"], chunks=chunks)
+    assert np.array_equal(walk["status"], direct["status"])
+    for k in ("r", "th", "ph", "t", "sig"):
+        assert np.array_equal(np.isnan(walk[k]), np.isnan(direct[k])), k
+        e = errors(walk[k], direct[k], k)
+        assert np.nanmax(e) < 5e-12, (k, float(np.nanmax(e)))
