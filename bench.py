@@ -272,13 +272,35 @@ def bench_equatorial(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_copy(args):
+    """A plain device-to-device copy of known size (1 GiB read + 1 GiB
+    written per launch): the reference point that scales the FETCH_SIZE and
+    WRITE_SIZE counters into bytes for the traffic figure."""
+    import torch
+    dev = torch.device("cuda", 0)
+    n = 1 << 27  # float64 elements = 1 GiB
+    x = torch.ones(n, device=dev, dtype=torch.float64)
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "device copy GB/s (counter calibration)", "value": 2 * n * 8 / dt / 1e9,
+                      "unit": "GB/s", "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                      "steps": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial"), default="raytrace")
+    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial", "copy"), default="raytrace")
     ap.add_argument("--retune", action="store_true", help="batch mode: re-run the autotuner")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -290,6 +312,8 @@ def main():
         return bench_batch(args)
     if args.mode == "equatorial":
         return bench_equatorial(args)
+    if args.mode == "copy":
+        return bench_copy(args)
 
     import torch
     from kgeo_amd import _lib

@@ -199,7 +199,7 @@ const char* kgeo_equatorial_last_error(void);
  * from unfinished ones.
  */
 typedef struct kgeo_batch_tuning {
-  int32_t samples_per_item; /* Mino-time samples per work item; a work item is 64 pixels x this */
+  int32_t samples_per_item; /* Mino-time samples per work item (rounded up to a multiple of 16); an item is 64 pixels x this */
   int32_t waves_per_cu;     /* persistent grid: CUs x this waves (1..64) */
   int32_t reserved[2];
 } kgeo_batch_tuning;

@@ -133,7 +133,7 @@ void kgh_raytrace_loops(const double* sc, const double* alpha, const double* bet
   }
   for (int64_t i = 0; i < npix; ++i) {
     for (int ch = 0; ch < chunks; ++ch) {
-      const int64_t j0 = (G * ch) / chunks, j1 = (G * (ch + 1)) / chunks;
+      const int64_t j0 = chunk_start(G, ch, chunks), j1 = chunk_start(G, ch + 1, chunks);
       uint32_t bits;
       if (do_phi_t) {
         bits = theta_pixel<true>(plan.data(), npix, i, gflags, o, i, j0, j1);

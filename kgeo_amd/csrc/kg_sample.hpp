@@ -13,9 +13,9 @@
 //
 // On the linspace grid the loops walk: each sample's state follows from the
 // previous one by the addition theorems (kg_step.hpp), with a direct
-// evaluation at the chunk's first sample, every kAnchorEvery samples, at the
-// last grid sample (tau = stop exactly) and wherever a step is not well
-// conditioned.  With caller-given Mino times (tau_in) every sample is direct.
+// evaluation at the chunk's first sample, at every multiple of kAnchorEvery,
+// at the last grid sample (tau = stop exactly) and wherever a step is not
+// well conditioned.  With caller-given Mino times (tau_in) every sample is direct.
 // Host-compilable (KG_INLINE), so the CPU test build runs the same loops.
 
 #pragma once
@@ -139,13 +139,12 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<double>& tc
   }
   WalkState<1> st;
   bool bad = false;
-  int since = 0;
-  for (int64_t j = j0; j < j1; ++j, ++since) {
+  for (int64_t j = j0; j < j1; ++j) {
     const int64_t idx = j * o.npix + i;
     const double tau = mino_tau(j, o.ngeo, stop, false);
     const double snarg = theta_arg(tc, tau);
     ThState<double> ts;
-    bool direct = (j == j0) || (since == kAnchorEvery) || (j == o.ngeo - 1);
+    bool direct = (j == j0) || (j % kAnchorEvery == 0) || (j == o.ngeo - 1);
     if (!direct) {
       WalkState<1> nx;
       if (walk_step<1, DO_PHI_T>(st, w, nx)) {
@@ -167,7 +166,6 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<double>& tc
       st.d = ts.d;
       st.D = DO_PHI_T ? ts.EmF / m : 0.0;
       st.P[0] = DO_PHI_T ? ts.Pi : 0.0;
-      since = 0;
     }
     double th = 0.0, Gph = 0.0, Gt = 0.0;
     theta_out<double, KIND>(tc, snarg, ts, DO_PHI_T, th, Gph, Gt);
@@ -199,14 +197,13 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<double>& 
   }
   WalkState<3> st;
   bool bad = false;
-  int since = 0;
-  for (int64_t j = j0; j < j1; ++j, ++since) {
+  for (int64_t j = j0; j < j1; ++j) {
     const int64_t idx = j * o.npix + i;
     const double tau = mino_tau(j, o.ngeo, stop, false);
     const double dv = rc.v[R_Xs] * tau;
     const double X = rc.v[R_U0] - dv;
     RState<double> rs;
-    bool direct = (j == j0) || (since == kAnchorEvery) || (j == o.ngeo - 1);
+    bool direct = (j == j0) || (j % kAnchorEvery == 0) || (j == o.ngeo - 1);
     if (!direct) {
       WalkState<3> nx;
       if (walk_step<3, DO_PHI_T>(st, w, nx)) {
@@ -230,12 +227,22 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<double>& 
       st.D = DO_PHI_T ? rs.EmF / m : 0.0;
 #pragma unroll
       for (int q = 0; q < 3; ++q) st.P[q] = DO_PHI_T ? rs.Pi[q] : 0.0;
-      since = 0;
     }
     RadialOut<double> ro = radial_out<double, CASE>(rc, tau, rs, DO_PHI_T);
     bad |= radial_store<DO_PHI_T>(o, cmb, idx, tau, ro);
   }
   return bad;
+}
+
+// First sample of chunk c of `chunks` over ngeo samples.  Chunk starts are
+// multiples of kAnchorEvery, so the walk's direct evaluations fall on the
+// same samples (multiples of kAnchorEvery, the last sample, and the
+// ill-conditioned steps) for every launch shape: results are bit-identical
+// whatever the chunking.
+KG_INLINE int64_t chunk_start(int64_t ngeo, int64_t c, int64_t chunks) {
+  if (c <= 0) return 0;
+  if (c >= chunks) return ngeo;
+  return ((ngeo * c) / chunks) / kAnchorEvery * kAnchorEvery;
 }
 
 // Walking needs the linspace grid with at least one interior step: not for

@@ -148,8 +148,8 @@ __global__ __launch_bounds__(kBlock) void kg_batch_persistent(BatchDev d) {
     const int64_t i = tile * kWave + lane;
     if (i < B.npix) {
       const int64_t s = B.slot_off + i;
-      const int64_t j0 = (B.ngeo * chunk) / B.chunks;
-      const int64_t j1 = (B.ngeo * (chunk + 1)) / B.chunks;
+      const int64_t j0 = chunk_start(B.ngeo, chunk, B.chunks);
+      const int64_t j1 = chunk_start(B.ngeo, chunk + 1, B.chunks);
       const uint32_t fl = d.flags[im];
       const bool phi_t = B.do_phi_t != 0;
       SampleOut o{nullptr, nullptr, phi_t ? d.t + B.out_off : nullptr, d.r + B.out_off, d.th + B.out_off,
@@ -263,7 +263,9 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
     B.slot_off = slots;
     B.out_off = outo;
     B.item_off = items;
-    const int64_t ch = (p.ngeo + tn.samples_per_item - 1) / tn.samples_per_item;
+    // items hold whole anchor intervals (chunk starts are multiples of kAnchorEvery)
+    const int64_t spi = (static_cast<int64_t>(tn.samples_per_item) + kAnchorEvery - 1) / kAnchorEvery * kAnchorEvery;
+    const int64_t ch = (p.ngeo + spi - 1) / spi;
     B.chunks = static_cast<int32_t>(ch);
     B.do_phi_t = (p.flags & KGEO_DO_PHI_T) ? 1 : 0;
     any_phi |= B.do_phi_t != 0;

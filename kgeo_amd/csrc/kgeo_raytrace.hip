@@ -102,9 +102,8 @@ template <bool DO_PHI_T>
 __global__ __launch_bounds__(kBlock) void kg_theta(DevParams p, SampleArgs a) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= p.npix) return;
-  const int64_t G = p.ngeo;
-  const int64_t j0 = (G * blockIdx.y) / a.chunks;
-  const int64_t j1 = (G * (blockIdx.y + 1)) / a.chunks;
+  const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
+  const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
   const uint32_t st = theta_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
   if (a.status && st) atomicOr(&a.status[i], st);
 }
@@ -113,9 +112,8 @@ template <bool DO_PHI_T>
 __global__ __launch_bounds__(kBlock) void kg_radial(DevParams p, SampleArgs a) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= p.npix) return;
-  const int64_t G = p.ngeo;
-  const int64_t j0 = (G * blockIdx.y) / a.chunks;
-  const int64_t j1 = (G * (blockIdx.y + 1)) / a.chunks;
+  const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
+  const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
   const uint32_t st = radial_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
   if (a.status && st) atomicOr(&a.status[i], st);
 }
@@ -180,13 +178,16 @@ __global__ __launch_bounds__(kBlock) void kg_counts(DevParams p, const double* _
   }
 }
 
-// Samples per lane chunking: enough (pixel, chunk) lanes to cover the chip
-// (256 CUs x 8 waves of 64) while keeping chunks >= 16 samples.
+// Samples per lane chunking: split ngeo so that about 2^20 (pixel, chunk)
+// lanes exist (256 CUs x 64 waves of 64 over the kernel's lifetime; more
+// lanes than the chip holds at once lets waves that finish early be
+// replaced), keeping chunks >= 32 samples so the walk's per-chunk set-up
+// (two direct evaluations) stays a few percent of the work.
 int pick_chunks(int64_t npix, int64_t ngeo) {
-  const int64_t target_lanes = 256LL * 8 * 64;
+  const int64_t target_lanes = 256LL * 64 * 64;
   int64_t c = (target_lanes + npix - 1) / (npix > 0 ? npix : 1);
   if (c < 1) c = 1;
-  const int64_t max_c = ngeo / 16 > 0 ? ngeo / 16 : 1;
+  const int64_t max_c = ngeo / 32 > 0 ? ngeo / 32 : 1;
   if (c > max_c) c = max_c;
   if (c > 65535) c = 65535;
   return static_cast<int>(c);

