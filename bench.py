@@ -54,10 +54,12 @@ def screen():
     return A.flatten(), B.flatten()
 
 
-def algorithmic_flops_per_sample(A, B, th, npix_sample=512, seed=0):
-    """Counts the per-sample FP64 operations of the kernel's own algorithm on a
-    seeded subset of the workload's pixels (CPU instantiation of the device
-    source, kgeo_amd/libkgeo_opcount.so)."""
+def algorithmic_flops_per_sample(A, B, th, chunks, npix_sample=512, seed=0):
+    """Counts the per-sample FP64 operations of the kernels' own loops (the
+    walk along the grid with its direct evaluations, split into `chunks`
+    ranges per pixel as the launch does) on a seeded subset of the workload's
+    pixels: CPU instantiation of the device source on an op-counting type,
+    kgeo_amd/libkgeo_opcount.so."""
     path = os.path.join(ROOT, "kgeo_amd", "libkgeo_opcount.so")
     lib = ctypes.CDLL(path)
     n = lib.kgc_num_ops()
@@ -68,7 +70,7 @@ def algorithmic_flops_per_sample(A, B, th, npix_sample=512, seed=0):
     ps = np.zeros(n, np.uint64)
     su = np.zeros(n, np.uint64)
     P = lambda x, t=ctypes.c_double: x.ctypes.data_as(ctypes.POINTER(t))
-    lib.kgc_count(P(sc), P(a_), P(b_), ctypes.c_int64(npix_sample), ctypes.c_int64(NGEO),
+    lib.kgc_count(P(sc), P(a_), P(b_), ctypes.c_int64(npix_sample), ctypes.c_int64(NGEO), ctypes.c_int(chunks),
                   P(ps, ctypes.c_uint64), P(su, ctypes.c_uint64))
     per = {k: int(v) / (npix_sample * NGEO) for k, v in zip(OP_NAMES, ps)}
     setup = {k: int(v) / npix_sample for k, v in zip(OP_NAMES, su)}
@@ -373,7 +375,8 @@ def main():
     st = status.cpu().numpy()
     nonfinite = int(np.sum(st & 1))
 
-    flops_s, flops_setup, ops = algorithmic_flops_per_sample(A, B, th)
+    chunks = int(lib.kgeo_sample_chunks(npix, NGEO))
+    flops_s, flops_setup, ops = algorithmic_flops_per_sample(A, B, th, chunks)
     achieved_tf = flops_s * samples / (samp_ms * 1e-3) / 1e12
     bytes_per_launch = 32.0 * samples
     hbm_gbs = bytes_per_launch / (samp_ms * 1e-3) / 1e9

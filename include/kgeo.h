@@ -113,6 +113,13 @@ int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, do
                 size_t workspace_bytes, void* stream);
 
 /*
+ * No counterpart in the reference.  The number of Mino-time chunks per pixel
+ * that kgeo_sample / kgeo_raytrace_ana split ngeo into for npix pixels (the
+ * launch shape; results do not depend on it).  For measurement tools.
+ */
+int kgeo_sample_chunks(int64_t npix, int64_t ngeo);
+
+/*
  * Replaces kerr_raytracing_ana.coords_at_tau (kerr_raytracing_ana.py:126-193):
  * coordinates at caller-given Mino times tau_in [ntau][npix].  Outputs as in
  * kgeo_raytrace_ana with ngeo = ntau (p->ngeo is ignored).

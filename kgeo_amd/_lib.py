@@ -56,6 +56,7 @@ _PROTOS = {
     "kgeo_last_error": (ctypes.c_char_p, []),
     "kgeo_fill_params": (None, [ctypes.POINTER(RayParams)]),
     "kgeo_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
+    "kgeo_sample_chunks": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64]),
     "kgeo_raytrace_ana": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 14
                           + [ctypes.c_size_t, ctypes.c_void_p]),
     "kgeo_coords_at_tau": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_void_p,

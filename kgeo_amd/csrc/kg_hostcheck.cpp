@@ -200,16 +200,21 @@ extern "C" {
 
 // One addition-theorem step of kg_step.hpp for a single third-kind
 // integral: in = {sn, cn, dn, (E-F)/m, Pi} at u and at v; out = the same at
-// u + v.  Returns 1 when the step is taken, 0 when it is refused.
-int kgh_walk_step(const double* at_u, const double* at_v, double m, double n, double* out) {
-  WalkState<1> a, r;
-  WalkStep<1> w;
-  a.s = at_u[0]; a.c = at_u[1]; a.d = at_u[2]; a.D = at_u[3]; a.P[0] = at_u[4];
-  w.b.s = at_v[0]; w.b.c = at_v[1]; w.b.d = at_v[2]; w.b.D = at_v[3]; w.b.P[0] = at_v[4];
+// u + v.  Characteristics the walk transforms (n > 1) are converted on the
+// way in and out as the sample loops do.  Returns 1 when the step is taken,
+// 0 when it is refused.
+int kgh_walk_step(const double* at_u, const double* at_v, double u, double v, double m, double n, double* out) {
+  WalkState<double, 1> a, r;
+  WalkStep<double, 1> w;
   const double nn[1] = {n};
   walk_init(w, m, nn);
-  const bool ok = walk_step<1, true>(a, w, r);
-  out[0] = r.s; out[1] = r.c; out[2] = r.d; out[3] = r.D; out[4] = r.P[0];
+  a.s = at_u[0]; a.c = at_u[1]; a.d = at_u[2]; a.D = at_u[3];
+  a.P[0] = walked_pi(w, 0, at_u[4], u, a.s, a.c, a.d);
+  w.b.s = at_v[0]; w.b.c = at_v[1]; w.b.d = at_v[2]; w.b.D = at_v[3];
+  w.b.P[0] = walked_pi(w, 0, at_v[4], v, w.b.s, w.b.c, w.b.d);
+  const bool ok = walk_step<double, 1, true>(a, w, r);
+  out[0] = r.s; out[1] = r.c; out[2] = r.d; out[3] = r.D;
+  out[4] = original_pi(w, 0, r.P[0], u + v, r.s, r.c, r.d);
   return ok ? 1 : 0;
 }
 

@@ -58,80 +58,64 @@ inline CD kpow(const CD& x, double y) { ++g_ops[POW]; return CD(std::pow(x.v, y)
 }  // namespace kgcount
 
 #include "kg_geodesic.hpp"
+#include "kg_sample.hpp"
 
 using kgcount::CD;
 using namespace kg;
 
-namespace {
-
-template <class A, class B> void copy_consts(const PixelConsts<A>& s, PixelConsts<B>& d) {
-  for (int i = 0; i < TH_NSLOT; ++i) d.th.v[i] = B(s.th.v[i]);
-  for (int i = 0; i < R_NSLOT; ++i) d.r.v[i] = B(s.r.v[i]);
-  d.th.kind = s.th.kind;
-  d.th.eta_zero = s.th.eta_zero;
-  d.r.rcase = s.r.rcase;
-  d.stop = B(s.stop);
-  d.cmb.fac_ph = B(s.cmb.fac_ph); d.cmb.cp = B(s.cmb.cp); d.cmb.cm = B(s.cmb.cm);
-  d.cmb.fac_t = B(s.cmb.fac_t); d.cmb.tp = B(s.cmb.tp); d.cmb.tm = B(s.cmb.tm);
-  d.cmb.a2 = B(s.cmb.a2); d.cmb.lam = B(s.cmb.lam);
-  const JacobiLadder<A>* ls[2] = {&s.lad_th, &s.lad_r};
-  JacobiLadder<B>* ld[2] = {&d.lad_th, &d.lad_r};
-  for (int q = 0; q < 2; ++q) {
-    for (int i = 0; i < kMaxLadder; ++i) ld[q]->k[i] = B(ls[q]->k[i]);
-    ld[q]->aN = B(ls[q]->aN);
-    ld[q]->K = B(ls[q]->K);
-    ld[q]->n = ls[q]->n;
-    ld[q]->capped = ls[q]->capped;
-  }
-}
-
-}  // namespace
 
 extern "C" {
 
 int kgc_num_ops(void) { return kgcount::NOPS; }
 
-// Per-sample op counts summed over npix pixels x ngeo linspace samples
-// (plus, separately, the per-pixel setup counts).  sc as in kg_hostcheck.cpp.
-void kgc_count(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo,
+// Per-sample op counts summed over npix pixels x ngeo linspace samples, as
+// the sample kernels execute them: the same per-pixel loops (kg_sample.hpp:
+// walking with direct evaluations at chunk starts, anchors, the last sample
+// and refused steps) with ngeo split into `chunks` ranges per pixel as the
+// kernels' grid does.  Separately, the per-pixel setup counts.  sc as in
+// kg_hostcheck.cpp.
+void kgc_count(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo, int chunks,
                uint64_t* per_sample_ops, uint64_t* setup_ops) {
   RayScalars<double> c = make_scalars(sc[0], sc[2], sc[1], sc[3]);
   c.a2 = sc[4]; c.sin_th = sc[5]; c.cos_th = sc[6]; c.cos2_th = sc[7]; c.rplus = sc[8]; c.rminus = sc[9];
-  std::vector<double> plan(PL_N);
+  RayScalars<CD> cc;
+  cc.a = CD(c.a); cc.a2 = CD(c.a2); cc.sin_th = CD(c.sin_th); cc.cos_th = CD(c.cos_th);
+  cc.cos2_th = CD(c.cos2_th); cc.rplus = CD(c.rplus); cc.rminus = CD(c.rminus);
+  cc.r_o = c.r_o; cc.maxtaufrac = c.maxtaufrac; cc.h = c.h; cc.minspin = c.minspin;
+  cc.spin_gt_min = c.spin_gt_min; cc.r_o_inf = c.r_o_inf;
+  std::vector<double> plan(static_cast<size_t>(npix) * PL_N);
   uint64_t setup_acc[kgcount::NOPS] = {0}, samp_acc[kgcount::NOPS] = {0};
+  uint32_t gflags = 0;
   for (int64_t i = 0; i < npix; ++i) {
-    PixelSetup<double> px = plan_pixel(c, alpha[i], beta[i], true, plan.data());
-    (void)px;
+    double pl[PL_N];
+    PixelSetup<double> px = plan_pixel(c, alpha[i], beta[i], true, pl);
+    for (int s = 0; s < PL_N; ++s) plan[static_cast<size_t>(s) * npix + i] = pl[s];
+    if (px.eta > 0.0) gflags |= GF_ANY_ETA_POS;
+    if (ngeo > 1 && pl[PL_STOP] / static_cast<double>(ngeo - 1) == 0.0) gflags |= GF_ANY_STEP_ZERO;
     // setup cost: theta/radial constants and ladders, counted on the CD type
     std::memset(kgcount::g_ops, 0, sizeof(kgcount::g_ops));
-    RayScalars<CD> cc;
-    cc.a = CD(c.a); cc.a2 = CD(c.a2); cc.sin_th = CD(c.sin_th); cc.cos_th = CD(c.cos_th);
-    cc.cos2_th = CD(c.cos2_th); cc.rplus = CD(c.rplus); cc.rminus = CD(c.rminus);
-    cc.r_o = c.r_o; cc.maxtaufrac = c.maxtaufrac; cc.h = c.h; cc.minspin = c.minspin;
-    cc.spin_gt_min = c.spin_gt_min; cc.r_o_inf = c.r_o_inf;
     PixelSetup<CD> pxc;
     pixel_setup(cc, CD(alpha[i]), CD(beta[i]), pxc);
     ThetaConsts<CD> tcc;
     RadialConsts<CD> rcc;
     theta_consts(cc, pxc, true, tcc);
     radial_consts(cc, pxc, true, rcc);
-    JacobiLadder<CD> l1, l2;
-    jacobi_ladder(tcc.v[TH_mp], l1);
-    jacobi_ladder(rcc.v[R_k], l2);
     for (int q = 0; q < kgcount::NOPS; ++q) setup_acc[q] += kgcount::g_ops[q];
-    // per-sample cost
-    PixelConsts<double> pc;
-    plan_unpack(c, plan.data(), pc);
-    PixelConsts<CD> pcc;
-    copy_consts(pc, pcc);
-    std::memset(kgcount::g_ops, 0, sizeof(kgcount::g_ops));
-    for (int64_t j = 0; j < ngeo; ++j) {
-      double tj = mino_tau(j, ngeo, pc.stop, false);
-      Sample<CD> s = eval_sample(pcc, CD(tj), true, true);
-      (void)s;
-    }
-    for (int q = 0; q < kgcount::NOPS; ++q) samp_acc[q] += kgcount::g_ops[q];
   }
+  // per-sample cost: the kernels' loops on the counting type
+  std::vector<double> t(static_cast<size_t>(ngeo) * npix), r(t.size()), th(t.size()), ph(t.size());
+  SampleOut o{nullptr, nullptr, t.data(), r.data(), th.data(), ph.data(), nullptr, npix, ngeo};
+  if (chunks < 1) chunks = 1;
+  std::memset(kgcount::g_ops, 0, sizeof(kgcount::g_ops));
+  for (int64_t i = 0; i < npix; ++i) {
+    for (int ch = 0; ch < chunks; ++ch) {
+      const int64_t j0 = chunk_start(ngeo, ch, chunks), j1 = chunk_start(ngeo, ch + 1, chunks);
+      if (j0 >= j1) continue;
+      theta_pixel<true, CD>(plan.data(), npix, i, gflags, o, i, j0, j1);
+      radial_pixel<true, CD>(cc, plan.data(), npix, i, gflags, o, i, j0, j1);
+    }
+  }
+  for (int q = 0; q < kgcount::NOPS; ++q) samp_acc[q] += kgcount::g_ops[q];
   for (int q = 0; q < kgcount::NOPS; ++q) {
     per_sample_ops[q] = samp_acc[q];
     setup_ops[q] = setup_acc[q];

@@ -43,170 +43,160 @@ KG_INLINE double sample_tau(const SampleOut& o, int64_t j, int64_t idx, double s
   return o.tau_in ? o.tau_in[idx] : mino_tau(j, o.ngeo, stop, any_step_zero);
 }
 
-template <int KIND, bool DO_PHI_T>
-KG_INLINE bool theta_loop(const SampleOut& o, const ThetaConsts<double>& tc, const JacobiLadder<double>& lad,
-                                  int64_t i, int64_t j0, int64_t j1, double stop, bool any_step_zero) {
-  bool bad = false;
-  for (int64_t j = j0; j < j1; ++j) {
-    const int64_t idx = j * o.npix + i;
-    const double tau = sample_tau(o, j, idx, stop, any_step_zero);
-    double th = 0.0, Gph = 0.0, Gt = 0.0;
-    if (KIND != THK_NONE) theta_eval<double, KIND>(tc, lad, tau, DO_PHI_T, th, Gph, Gt);
-    if (o.tau_out) o.tau_out[idx] = tau;
-    if (o.th) o.th[idx] = th;
-    bad |= !kfinite(th);
-    if (DO_PHI_T) {
-      o.ph[idx] = Gph;
-      o.t[idx] = Gt;
-    }
-  }
-  return bad;
-}
-
-template <int CASE, bool DO_PHI_T>
-KG_INLINE bool radial_loop(const SampleOut& o, const RadialConsts<double>& rc, const JacobiLadder<double>& lad,
-                                   const CombineConsts<double>& cmb, int64_t i, int64_t j0, int64_t j1, double stop,
-                                   bool any_step_zero) {
-  bool bad = false;
-  for (int64_t j = j0; j < j1; ++j) {
-    const int64_t idx = j * o.npix + i;
-    const double tau = sample_tau(o, j, idx, stop, any_step_zero);
-    RadialOut<double> ro = radial_eval<double, CASE>(rc, lad, tau, DO_PHI_T);
-    if (o.r) o.r[idx] = ro.r;
-    bad |= !kfinite(ro.r);
-    if (DO_PHI_T) {
-      const double Gph = o.ph[idx];
-      const double Gt = o.t[idx];
-      Sample<double> s;
-      combine(cmb, tau, ro, 0.0, Gph, Gt, s);
-      o.t[idx] = s.t;
-      o.ph[idx] = s.ph;
-      if (o.sig) o.sig[idx] = s.sig;
-      bad |= !(kfinite(s.t) && kfinite(s.ph) && kfinite(s.sig));
-    }
-  }
-  return bad;
-}
+// The loops below are templated on the scalar type: T = double in the
+// kernels; the op-counting CPU build instantiates them on its counting type
+// (outputs are stored through val()).
 
 // Writers shared by the direct and the walking loops.
-template <bool DO_PHI_T>
-KG_INLINE bool theta_store(const SampleOut& o, int64_t idx, double tau, double th, double Gph, double Gt) {
+template <class T, bool DO_PHI_T>
+KG_INLINE bool theta_store(const SampleOut& o, int64_t idx, double tau, const T& th, const T& Gph, const T& Gt) {
   if (o.tau_out) o.tau_out[idx] = tau;
-  if (o.th) o.th[idx] = th;
+  if (o.th) o.th[idx] = val(th);
   if (DO_PHI_T) {
-    o.ph[idx] = Gph;
-    o.t[idx] = Gt;
+    o.ph[idx] = val(Gph);
+    o.t[idx] = val(Gt);
   }
-  return !kfinite(th);
+  return !kfinite(val(th));
 }
 
-template <bool DO_PHI_T>
-KG_INLINE bool radial_store(const SampleOut& o, const CombineConsts<double>& cmb, int64_t idx, double tau,
-                            const RadialOut<double>& ro) {
-  if (o.r) o.r[idx] = ro.r;
-  bool bad = !kfinite(ro.r);
+template <class T, bool DO_PHI_T>
+KG_INLINE bool radial_store(const SampleOut& o, const CombineConsts<T>& cmb, int64_t idx, double tau,
+                            const RadialOut<T>& ro) {
+  if (o.r) o.r[idx] = val(ro.r);
+  bool bad = !kfinite(val(ro.r));
   if (DO_PHI_T) {
-    const double Gph = o.ph[idx];
-    const double Gt = o.t[idx];
-    Sample<double> s;
-    combine(cmb, tau, ro, 0.0, Gph, Gt, s);
-    o.t[idx] = s.t;
-    o.ph[idx] = s.ph;
-    if (o.sig) o.sig[idx] = s.sig;
-    bad |= !(kfinite(s.t) && kfinite(s.ph) && kfinite(s.sig));
+    const T Gph = T(o.ph[idx]);
+    const T Gt = T(o.t[idx]);
+    Sample<T> s;
+    combine(cmb, T(tau), ro, T(0.0), Gph, Gt, s);
+    o.t[idx] = val(s.t);
+    o.ph[idx] = val(s.ph);
+    if (o.sig) o.sig[idx] = val(s.sig);
+    bad |= !(kfinite(val(s.t)) && kfinite(val(s.ph)) && kfinite(val(s.sig)));
+  }
+  return bad;
+}
+
+// Direct evaluation of every sample (caller-given Mino times, numpy's
+// zero-step grid, and the motion classes and cases the walk does not cover).
+template <class T, int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_loop(const SampleOut& o, const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, int64_t i,
+                          int64_t j0, int64_t j1, double stop, bool any_step_zero) {
+  bool bad = false;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t idx = j * o.npix + i;
+    const double tau = sample_tau(o, j, idx, stop, any_step_zero);
+    T th = T(0.0), Gph = T(0.0), Gt = T(0.0);
+    if (KIND != THK_NONE) theta_eval<T, KIND>(tc, lad, T(tau), DO_PHI_T, th, Gph, Gt);
+    bad |= theta_store<T, DO_PHI_T>(o, idx, tau, th, Gph, Gt);
+  }
+  return bad;
+}
+
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_loop(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
+                           const CombineConsts<T>& cmb, int64_t i, int64_t j0, int64_t j1, double stop,
+                           bool any_step_zero) {
+  bool bad = false;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t idx = j * o.npix + i;
+    const double tau = sample_tau(o, j, idx, stop, any_step_zero);
+    RadialOut<T> ro = radial_eval<T, CASE>(rc, lad, T(tau), DO_PHI_T);
+    bad |= radial_store<T, DO_PHI_T>(o, cmb, idx, tau, ro);
   }
   return bad;
 }
 
 // Theta samples j0..j1-1 on the linspace grid (ordinary or vortical motion),
 // walking u_j = w (s_G - j h) with the addition theorems.
-template <int KIND, bool DO_PHI_T>
-KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<double>& tc, const JacobiLadder<double>& lad,
-                               int64_t i, int64_t j0, int64_t j1, double stop) {
-  const double m = tc.v[TH_k];
+template <class T, int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, int64_t i,
+                               int64_t j0, int64_t j1, double stop) {
+  const T m = tc.v[TH_k];
   const double h = stop / static_cast<double>(o.ngeo - 1);
-  WalkStep<1> w;
+  WalkStep<T, 1> w;
   {
-    ThState<double> sb;
-    theta_state<double, KIND>(tc, lad, -tc.v[TH_w] * h, DO_PHI_T, sb);
+    ThState<T> sb;
+    const T du = -tc.v[TH_w] * h;
+    theta_state<T, KIND>(tc, lad, du, DO_PHI_T, sb);
+    const T n[1] = {tc.v[TH_n]};
+    walk_init(w, m, n);
     w.b.s = sb.s;
     w.b.c = sb.c;
     w.b.d = sb.d;
-    w.b.D = DO_PHI_T ? sb.EmF / m : 0.0;
-    w.b.P[0] = DO_PHI_T ? sb.Pi : 0.0;
-    const double n[1] = {tc.v[TH_n]};
-    walk_init(w, m, n);
+    w.b.D = DO_PHI_T ? sb.EmF / m : T(0.0);
+    w.b.P[0] = DO_PHI_T ? walked_pi(w, 0, sb.Pi, du, sb.s, sb.c, sb.d) : T(0.0);
   }
-  WalkState<1> st;
+  WalkState<T, 1> st;
   bool bad = false;
   for (int64_t j = j0; j < j1; ++j) {
     const int64_t idx = j * o.npix + i;
     const double tau = mino_tau(j, o.ngeo, stop, false);
-    const double snarg = theta_arg(tc, tau);
-    ThState<double> ts;
+    const T snarg = theta_arg(tc, T(tau));
+    ThState<T> ts;
     bool direct = (j == j0) || (j % kAnchorEvery == 0) || (j == o.ngeo - 1);
     if (!direct) {
-      WalkState<1> nx;
-      if (walk_step<1, DO_PHI_T>(st, w, nx)) {
+      WalkState<T, 1> nx;
+      if (walk_step<T, 1, DO_PHI_T>(st, w, nx)) {
         st = nx;
         ts.s = st.s;
         ts.c = st.c;
         ts.d = st.d;
         ts.F = snarg;
         ts.EmF = m * st.D;
-        ts.Pi = st.P[0];
+        ts.Pi = DO_PHI_T ? original_pi(w, 0, st.P[0], snarg, st.s, st.c, st.d) : T(0.0);
       } else {
         direct = true;
       }
     }
     if (direct) {
-      theta_state<double, KIND>(tc, lad, snarg, DO_PHI_T, ts);
+      theta_state<T, KIND>(tc, lad, snarg, DO_PHI_T, ts);
       st.s = ts.s;
       st.c = ts.c;
       st.d = ts.d;
-      st.D = DO_PHI_T ? ts.EmF / m : 0.0;
-      st.P[0] = DO_PHI_T ? ts.Pi : 0.0;
+      st.D = DO_PHI_T ? ts.EmF / m : T(0.0);
+      st.P[0] = DO_PHI_T ? walked_pi(w, 0, ts.Pi, snarg, ts.s, ts.c, ts.d) : T(0.0);
     }
-    double th = 0.0, Gph = 0.0, Gt = 0.0;
-    theta_out<double, KIND>(tc, snarg, ts, DO_PHI_T, th, Gph, Gt);
-    bad |= theta_store<DO_PHI_T>(o, idx, tau, th, Gph, Gt);
+    T th = T(0.0), Gph = T(0.0), Gt = T(0.0);
+    theta_out<T, KIND>(tc, snarg, ts, DO_PHI_T, th, Gph, Gt);
+    bad |= theta_store<T, DO_PHI_T>(o, idx, tau, th, Gph, Gt);
   }
   return bad;
 }
 
 // Radial samples j0..j1-1 on the linspace grid, walking X_j = U0 - Xs j h.
-template <int CASE, bool DO_PHI_T>
-KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<double>& rc, const JacobiLadder<double>& lad,
-                                const CombineConsts<double>& cmb, int64_t i, int64_t j0, int64_t j1, double stop) {
-  const double m = rc.v[R_k];
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
+                                const CombineConsts<T>& cmb, int64_t i, int64_t j0, int64_t j1, double stop) {
+  const T m = rc.v[R_k];
   const double h = stop / static_cast<double>(o.ngeo - 1);
-  WalkStep<3> w;
+  WalkStep<T, 3> w;
   {
-    const double dX = -rc.v[R_Xs] * h;
-    RState<double> sb;
-    radial_state<double, CASE>(rc, lad, dX, INFINITY, DO_PHI_T, sb);  // plain Jacobi at dX
+    const T dX = -rc.v[R_Xs] * h;
+    RState<T> sb;
+    radial_state<T, CASE>(rc, lad, dX, T(INFINITY), DO_PHI_T, sb);  // plain Jacobi at dX
+    T n[3], pc[3];
+    radial_pi_params<T, CASE>(rc, n, pc);
+    walk_init(w, m, n);
     w.b.s = sb.s;
     w.b.c = sb.c;
     w.b.d = sb.d;
-    w.b.D = DO_PHI_T ? sb.EmF / m : 0.0;
-    double n[3], pc[3];
-    radial_pi_params<double, CASE>(rc, n, pc);
+    w.b.D = DO_PHI_T ? sb.EmF / m : T(0.0);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) w.b.P[q] = DO_PHI_T ? sb.Pi[q] : 0.0;
-    walk_init(w, m, n);
+    for (int q = 0; q < 3; ++q) w.b.P[q] = DO_PHI_T ? walked_pi(w, q, sb.Pi[q], dX, sb.s, sb.c, sb.d) : T(0.0);
   }
-  WalkState<3> st;
+  WalkState<T, 3> st;
   bool bad = false;
   for (int64_t j = j0; j < j1; ++j) {
     const int64_t idx = j * o.npix + i;
     const double tau = mino_tau(j, o.ngeo, stop, false);
-    const double dv = rc.v[R_Xs] * tau;
-    const double X = rc.v[R_U0] - dv;
-    RState<double> rs;
+    const T dv = rc.v[R_Xs] * tau;
+    const T X = rc.v[R_U0] - dv;
+    RState<T> rs;
     bool direct = (j == j0) || (j % kAnchorEvery == 0) || (j == o.ngeo - 1);
     if (!direct) {
-      WalkState<3> nx;
-      if (walk_step<3, DO_PHI_T>(st, w, nx)) {
+      WalkState<T, 3> nx;
+      if (walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
         st = nx;
         rs.s = st.s;
         rs.c = st.c;
@@ -214,22 +204,22 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<double>& 
         rs.F = X;
         rs.EmF = m * st.D;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) rs.Pi[q] = st.P[q];
+        for (int q = 0; q < 3; ++q) rs.Pi[q] = DO_PHI_T ? original_pi(w, q, st.P[q], X, st.s, st.c, st.d) : T(0.0);
       } else {
         direct = true;
       }
     }
     if (direct) {
-      radial_state<double, CASE>(rc, lad, X, dv, DO_PHI_T, rs);
+      radial_state<T, CASE>(rc, lad, X, dv, DO_PHI_T, rs);
       st.s = rs.s;
       st.c = rs.c;
       st.d = rs.d;
-      st.D = DO_PHI_T ? rs.EmF / m : 0.0;
+      st.D = DO_PHI_T ? rs.EmF / m : T(0.0);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) st.P[q] = DO_PHI_T ? rs.Pi[q] : 0.0;
+      for (int q = 0; q < 3; ++q) st.P[q] = DO_PHI_T ? walked_pi(w, q, rs.Pi[q], X, rs.s, rs.c, rs.d) : T(0.0);
     }
-    RadialOut<double> ro = radial_out<double, CASE>(rc, tau, rs, DO_PHI_T);
-    bad |= radial_store<DO_PHI_T>(o, cmb, idx, tau, ro);
+    RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
+    bad |= radial_store<T, DO_PHI_T>(o, cmb, idx, tau, ro);
   }
   return bad;
 }
@@ -251,17 +241,17 @@ KG_INLINE bool walk_ok(const SampleOut& o, bool any_step_zero) {
   return o.tau_in == nullptr && !any_step_zero && o.ngeo > 2;
 }
 
-KG_INLINE uint32_t loop_status(bool bad, const JacobiLadder<double>& lad) {
+template <class T> KG_INLINE uint32_t loop_status(bool bad, const JacobiLadder<T>& lad) {
   return (bad ? ST_NONFINITE : 0u) | (lad.capped ? ST_LADDER_CAP : 0u);
 }
 
 // Theta part of samples j0..j1-1 of output pixel i, whose plan column is p.
-template <bool DO_PHI_T>
+template <bool DO_PHI_T, class T = double>
 KG_INLINE uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
-                                       const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
-  ThetaConsts<double> tc;
+                               const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
+  ThetaConsts<T> tc;
 #pragma unroll
-  for (int s = 0; s < TH_NSLOT; ++s) tc.v[s] = plan[static_cast<int64_t>(PL_TH + s) * stride + p];
+  for (int s = 0; s < TH_NSLOT; ++s) tc.v[s] = T(plan[static_cast<int64_t>(PL_TH + s) * stride + p]);
   const int kind = static_cast<int>(plan[static_cast<int64_t>(PL_THK) * stride + p]);
   const bool eta0 = plan[static_cast<int64_t>(PL_ETA0) * stride + p] != 0.0;
   const double stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
@@ -269,54 +259,53 @@ KG_INLINE uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, 
   const bool any_step_zero = (gflags & GF_ANY_STEP_ZERO) != 0;
   // eta == 0 pixels take theta = 0 when no pixel of the image has eta > 0
   const int k = (eta0 && !any_eta_pos) ? THK_NONE : kind;
-  JacobiLadder<double> lad;
+  JacobiLadder<T> lad;
   jacobi_ladder(tc.v[TH_mp], lad);
   const bool walk = walk_ok(o, any_step_zero);
   bool bad;
   switch (k) {
     case THK_ORD:
-      bad = walk ? theta_loop_walk<THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop)
-                 : theta_loop<THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero);
+      bad = walk ? theta_loop_walk<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop)
+                 : theta_loop<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero);
       break;
     case THK_VORT:
-      bad = walk ? theta_loop_walk<THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop)
-                 : theta_loop<THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero);
+      bad = walk ? theta_loop_walk<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop)
+                 : theta_loop<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero);
       break;
-    case THK_ORD_K0: bad = theta_loop<THK_ORD_K0, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
-    default: bad = theta_loop<THK_NONE, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    case THK_ORD_K0: bad = theta_loop<T, THK_ORD_K0, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
+    default: bad = theta_loop<T, THK_NONE, DO_PHI_T>(o, tc, lad, i, j0, j1, stop, any_step_zero); break;
   }
   return loop_status(bad, lad);
 }
 
 // Radial part (and the phi, t, sigma combination) of the same samples.
-template <bool DO_PHI_T>
-KG_INLINE uint32_t radial_pixel(const RayScalars<double>& c, const double* __restrict__ plan, int64_t stride,
-                                        int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j0,
-                                        int64_t j1) {
-  RadialConsts<double> rc;
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride, int64_t p,
+                                uint32_t gflags, const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
+  RadialConsts<T> rc;
 #pragma unroll
-  for (int s = 0; s < R_NSLOT; ++s) rc.v[s] = plan[static_cast<int64_t>(PL_R + s) * stride + p];
+  for (int s = 0; s < R_NSLOT; ++s) rc.v[s] = T(plan[static_cast<int64_t>(PL_R + s) * stride + p]);
   rc.rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
-  const double lam = plan[static_cast<int64_t>(PL_LAM) * stride + p];
+  const T lam = T(plan[static_cast<int64_t>(PL_LAM) * stride + p]);
   const double stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
   const bool any_step_zero = (gflags & GF_ANY_STEP_ZERO) != 0;
-  const CombineConsts<double> cmb = combine_consts(c, lam);
-  JacobiLadder<double> lad;
+  const CombineConsts<T> cmb = combine_consts(c, lam);
+  JacobiLadder<T> lad;
   jacobi_ladder(rc.v[R_k], lad);
   const bool walk = walk_ok(o, any_step_zero) && lad.n >= 0;
   bool bad;
   switch (rc.rcase) {
     case 4:
-      bad = walk ? radial_loop_walk<4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
-                 : radial_loop<4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
+      bad = walk ? radial_loop_walk<T, 4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
+                 : radial_loop<T, 4, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
       break;
     case 3:
-      bad = walk ? radial_loop_walk<3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
-                 : radial_loop<3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
+      bad = walk ? radial_loop_walk<T, 3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
+                 : radial_loop<T, 3, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
       break;
     default:
-      bad = walk ? radial_loop_walk<2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
-                 : radial_loop<2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
+      bad = walk ? radial_loop_walk<T, 2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop)
+                 : radial_loop<T, 2, DO_PHI_T>(o, rc, lad, cmb, i, j0, j1, stop, any_step_zero);
       break;
   }
   return loop_status(bad, lad);

@@ -234,9 +234,7 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   const uint32_t* gflags = static_cast<const uint32_t*>(ws);
   const double* plan = reinterpret_cast<const double*>(static_cast<const char*>(ws) + kFlagBytes);
   const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
-  int chunks = env_chunks();
-  if (chunks <= 0) chunks = pick_chunks(d.npix, ngeo);
-  if (chunks > ngeo) chunks = static_cast<int>(ngeo);
+  const int chunks = kgeo_sample_chunks(d.npix, ngeo);
   dim3 grid(nb, static_cast<unsigned>(chunks));
   SampleArgs a{plan, gflags, SampleOut{tau_in, tau, t, r, th, ph, sig, d.npix, ngeo}, status, chunks};
   if (d.do_phi_t) {
@@ -270,6 +268,14 @@ void kgeo_fill_params(kgeo_ray_params* p) {
   const double root = std::sqrt(1.0 - p->a2);
   p->rplus = 1.0 + root;
   p->rminus = 1.0 - root;
+}
+
+int kgeo_sample_chunks(int64_t npix, int64_t ngeo) {
+  if (npix <= 0 || ngeo <= 0) return 0;
+  int c = env_chunks();
+  if (c <= 0) c = pick_chunks(npix, ngeo);
+  if (c > ngeo) c = static_cast<int>(ngeo);
+  return c;
 }
 
 size_t kgeo_workspace_bytes(int64_t npix) {

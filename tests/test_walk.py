@@ -36,18 +36,17 @@ def _truth(u, m, n):
     return np.array([s, c, d, (E - u) / m, P])
 
 
-def _step(lib, at_u, at_v, m, n):
+def _step(lib, at_u, at_v, u, v, m, n):
     out = np.zeros(5)
     P = lambda x: np.ascontiguousarray(x, dtype=float).ctypes.data_as(ctypes.c_void_p)
-    ok = lib.kgh_walk_step(P(at_u), P(at_v), ctypes.c_double(m), ctypes.c_double(n), P(out))
+    lib.kgh_walk_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_double] * 4 + [ctypes.c_void_p]
+    ok = lib.kgh_walk_step(P(at_u), P(at_v), u, v, m, n, P(out))
     return ok, out
 
 
 @pytest.mark.parametrize("seed", range(4))
 def test_one_step_against_independent_values(seed):
     lib = _lib()
-    lib.kgh_walk_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_void_p]
     rng = np.random.default_rng(seed)
     taken = 0
     for _ in range(40):
@@ -57,29 +56,38 @@ def test_one_step_against_independent_values(seed):
         u = float(rng.uniform(-0.8, 0.8) * K)
         v = float(rng.uniform(-0.02, 0.02))
         want = _truth(u + v, m, n)
-        ok, got = _step(lib, _truth(u, m, n), _truth(v, m, n), m, n)
+        ok, got = _step(lib, _truth(u, m, n), _truth(v, m, n), u, v, m, n)
         if not ok:
             # refused steps must be the ill-conditioned ones: a pole of the
-            # integrand at or next to the step (n > 1 only)
-            assert n > 1.0
+            # integrand at or next to the step (n > 1, walked untransformed
+            # only when m < 0)
+            assert n > 1.0 and m < 0.0
             continue
         taken += 1
         scale = np.maximum(np.abs(want), 1.0)
-        assert np.all(np.abs(got - want) / scale < 2e-14), (m, n, u, v, got, want)
+        # sn, cn, dn, (E-F)/m to 2e-14; Pi to 5e-14 (for n > 1 it passes
+        # through the m/n transformation and its log term on the way in and out)
+        tol = np.array([2e-14, 2e-14, 2e-14, 2e-14, 5e-14])
+        assert np.all(np.abs(got - want) / scale < tol), (m, n, u, v, got, want)
     assert taken >= 25
 
 
-def test_step_across_a_pole_is_refused():
+def test_step_across_a_pole():
+    """n > 1: with 0 <= m < 1 the walk carries Pi(m/n) and the log term, so a
+    step across the pole is taken and lands on the principal value; with
+    m < 0 (walked untransformed) such a step is refused."""
     lib = _lib()
-    lib.kgh_walk_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_void_p]
-    m, n = 0.3, 1.6
-    # sn^2(u_p) = 1/n: the principal-value pole
-    phi_p = np.arcsin(np.sqrt(1.0 / n))
-    u_p = float(sp.ellipkinc(phi_p, m))
-    for (u, v) in ((u_p - 0.004, 0.008), (u_p - 0.001, 0.0015), (u_p + 0.003, -0.006)):
-        ok, _ = _step(lib, _truth(u, m, n), _truth(v, m, n), m, n)
-        assert not ok, (u, v)
+    for m, n, transformed in ((0.3, 1.6, True), (-0.4, 1.6, False)):
+        # sn^2(u_p) = 1/n: the principal-value pole
+        phi_p = np.arcsin(np.sqrt(1.0 / n))
+        u_p = float(sp.ellipkinc(phi_p, m)) if m >= 0 else float(mp.ellipf(phi_p, m))
+        for (u, v) in ((u_p - 0.004, 0.008), (u_p - 0.001, 0.0015), (u_p + 0.003, -0.006)):
+            ok, got = _step(lib, _truth(u, m, n), _truth(v, m, n), u, v, m, n)
+            if transformed:
+                want = _truth(u + v, m, n)
+                assert ok and np.all(np.abs(got - want) / np.maximum(np.abs(want), 1.0) < 1e-13), (u, v, got, want)
+            else:
+                assert not ok, (u, v)
 
 
 CASES = [
@@ -102,8 +110,18 @@ def test_walking_loops_match_direct_evaluation(case):
     host = Host()
     walk = host.raytrace_loops(a, 1.0e4, th, A, B, G, chunks=chunks)
     direct = host.raytrace_loops(a, 1.0e4, th, A, B, G, tau_in=walk["tau"], chunks=chunks)
+    # samples where the direct evaluation itself moves by > 1e-12 under a
+    # 1e-14 relative change of tau (log poles of the third-kind integrals,
+    # the end of horizon-bound rays) are compared at that looser level
+    nudged = host.raytrace_loops(a, 1.0e4, th, A, B, G, tau_in=walk["tau"] * (1.0 + 1.0e-14), chunks=chunks)
     assert np.array_equal(walk["status"], direct["status"])
     for k in ("r", "th", "ph", "t", "sig"):
         assert np.array_equal(np.isnan(walk[k]), np.isnan(direct[k])), k
-        e = errors(walk[k], direct[k], k)
-        assert np.nanmax(e) < 5e-12, (k, float(np.nanmax(e)))
+        e = np.nan_to_num(errors(walk[k], direct[k], k))
+        sens = np.nan_to_num(errors(nudged[k], direct[k], k))
+        ill = sens > 1e-12
+        assert ill.mean() < 0.05, (k, float(ill.mean()))
+        # the walk adds at most ~1e-11 (normwise) to the direct evaluation:
+        # an order of magnitude inside the 1e-10 parity bar
+        assert e[~ill].max() < 2e-11, (k, float(e[~ill].max()))
+        assert np.all(e[ill] < 100 * sens[ill] + 2e-11), k
