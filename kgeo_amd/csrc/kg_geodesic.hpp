@@ -378,6 +378,7 @@ KG_INLINE void theta_consts(const RayScalars<T>& c, const PixelSetup<T>& px, boo
     tc.v[TH_n] = up;
     tc.v[TH_A] = (-s) * ksqrt(up);  // theta = arccos(A sinPhi)
     tc.v[TH_B] = 2.0 * up * pref;   // G_t prefactor
+    tc.v[TH_C] = -tc.v[TH_B] / (2.0 * k);  // G_t = TH_C (E - F) - sGt for k != 0
     tc.kind = (val(k) == 0.0) ? THK_ORD_K0 : THK_ORD;
     if (do_phi_t) {
       T EmKk = ellip_EmK(k);
@@ -481,10 +482,9 @@ KG_INLINE void theta_out(const ThetaConsts<T>& tc, const T& snarg, const ThState
     th = kacos(v[TH_A] * st.s);
   }
   if (!do_phi_t) return;
-  const T& k = v[TH_k];
   Gph = v[TH_pref] * st.Pi - v[TH_sGph];
   if (kind == THK_ORD) {
-    Gt = -(v[TH_B] * st.EmF) / (2.0 * k) - v[TH_sGt];
+    Gt = v[TH_C] * st.EmF - v[TH_sGt];
   } else if (kind == THK_ORD_K0) {
     // k == 0: m' = 0, am(u|0) = u (kerr_raytracing_ana.py:290-291)
     T Phi = snarg;
@@ -519,14 +519,21 @@ enum RSlot {
   C2_rr1 = 8, C2_rr2, C2_rr3, C2_rr4, C2_rr31, C2_rr41, C2_rr43, C2_rrp3, C2_rrm3,
   C2_n1, C2_np, C2_nm, C2_Pc1, C2_Pcp, C2_Pcm, C2_Eb, C2_Pb1, C2_Pbp, C2_Pbm,
   C2_c, C2_fp, C2_fm, C2_sq, C2_dX, C2_drso, C2_ro3, C2_h12,
+  C2_irp3, C2_irm3, C2_Ho,  // 1/(r+ - r3), 1/(r- - r3), H at the observer
   // case 3 (kerr_raytracing_ana.py:473-552)
   C3_AA = 8, C3_BB, C3_rr1, C3_rr2, C3_al0, C3_alp, C3_alm, C3_nn0, C3_nnp, C3_nnm,
   C3_Pc0, C3_Pcp, C3_Pcm, C3_R1b0, C3_R2b0, C3_R1bp, C3_R1bm, C3_c0, C3_cp, C3_cm,
   C3_pr, C3_sqAB, C3_Ipd, C3_Imd, C3_hasm,
+  C3_p0, C3_pp, C3_pm,  // p1 of the three R1 blocks
+  C3_i0, C3_ip, C3_im,  // 1/(1 - al^2) of the three R1 blocks
+  C3_iIpd, C3_iImd,     // 1/Ipd, 1/Imd
   // case 4 (kerr_raytracing_ana.py:404-471)
   C4_a2 = 8, C4_b1, C4_g0, C4_gp, C4_gm, C4_n0, C4_np, C4_nm, C4_Pc0, C4_Pcp, C4_Pcm,
   C4_S1b0, C4_S2b0, C4_S1bp, C4_S1bm, C4_P1f, C4_P2f, C4_Ppf, C4_Pmf, C4_aa1, C4_Ipf, C4_Imf,
-  R_NSLOT = 35
+  C4_q0, C4_qp, C4_qm,  // p2 of the three S1 blocks
+  C4_l0, C4_lp, C4_lm,  // ln|(1 - p2)/(1 + p2)| of the three S1 blocks
+  C4_i0, C4_ip, C4_im,  // 1/(1 + al^2) of the three S1 blocks
+  R_NSLOT = 41
 };
 
 template <class T> struct RadialConsts {
@@ -555,6 +562,27 @@ KG_INLINE T R2_block(const T& al, const T& nn, const T& R1, const T& F, const T&
 
 // S1/S2 building blocks (kerr_raytracing_ana.py:638-663).  tan(phi) enters
 // only through (al - tan)/(1 + al tan) = (al c - s)/(c + al s).
+// The R1 and S1 blocks with their per-pixel p and the log of their f term
+// supplied: lg = ln|(p1 dn + sn)/(p1 dn - sn)| for R1 and
+// lg = ln|(1 + p2 dn)/(1 - p2 dn)|, lc = ln|(1 - p2)/(1 + p2)| for S1 (the
+// f1, f2 of R1_block and S1_block).  The sample loops take lg from a direct
+// log or from its walk.
+template <class T> KG_INLINE T R1_from(const T& al, const T& Pi, const T& p1, const T& lg, const T& inv1mal2) {
+  return (Pi - al * (0.5 * p1 * lg)) * inv1mal2;
+}
+template <class T>
+KG_INLINE T S1_from(const T& al, const T& Pi, const T& F, const T& p2, const T& lg, const T& lc, const T& inv1pal2) {
+  return (F + (al * al) * Pi - al * (0.5 * p2 * (lg + lc))) * inv1pal2;
+}
+template <class T> KG_INLINE T R1_p(const T& al, const T& j) {
+  T al2 = al * al;
+  return ksqrt((al2 - 1.0) / (j + (1.0 - j) * al2));
+}
+template <class T> KG_INLINE T S1_p(const T& al, const T& j) {
+  T al2 = al * al;
+  return ksqrt((1.0 + al2) / (1.0 - j + al2));
+}
+
 template <class T>
 KG_INLINE T S1_block(const T& al, const T& Pi, const T& F, const T& s2phi, const T& j) {
   T al2 = al * al;
@@ -660,6 +688,14 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C4_S2b0] = S2_block(g0, S1b0, Fb, Eb, sb, cb, s2b, k4);
       v[C4_S1bp] = S1_block(gp, ellip_pi_amp(Lb, np, k4, Pcp), Fb, s2b, k4);
       v[C4_S1bm] = S1_block(gm, ellip_pi_amp(Lb, nm, k4, Pcm), Fb, s2b, k4);
+      const T q0 = S1_p(g0, k4), qp = S1_p(gp, k4), qm = S1_p(gm, k4);
+      v[C4_q0] = q0; v[C4_qp] = qp; v[C4_qm] = qm;
+      v[C4_l0] = klog(kfabs((1.0 - q0) / (1.0 + q0)));
+      v[C4_lp] = klog(kfabs((1.0 - qp) / (1.0 + qp)));
+      v[C4_lm] = klog(kfabs((1.0 - qm) / (1.0 + qm)));
+      v[C4_i0] = 1.0 / (1.0 + g0 * g0);
+      v[C4_ip] = 1.0 / (1.0 + gp * gp);
+      v[C4_im] = 1.0 / (1.0 + gm * gm);
       T aa0 = (a2 / g0) * (1.0 + g0 * g0);
       v[C4_P1f] = pref * aa0;
       v[C4_P2f] = pref * (aa0 * aa0);
@@ -727,6 +763,12 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C3_R1bp] = R1_block(alp, ellip_pi_amp(Lb, nnp, k3, Pcp), sb, s2b, k3);
       v[C3_R1bm] = c.spin_gt_min ? R1_block(alm, ellip_pi_amp(Lb, nnm, k3, Pcm), sb, s2b, k3) : T(0.0);
       v[C3_hasm] = c.spin_gt_min ? T(1.0) : T(0.0);
+      v[C3_p0] = R1_p(al0, k3);
+      v[C3_pp] = R1_p(alp, k3);
+      v[C3_pm] = R1_p(alm, k3);
+      v[C3_i0] = 1.0 / (1.0 - al0 * al0);
+      v[C3_ip] = 1.0 / (1.0 - alp * alp);
+      v[C3_im] = 1.0 / (1.0 - alm * alm);
       T sqAB = ksqrt(AA * BB);
       T c0 = (2.0 * rr21 * sqAB) / (BB * BB - AA * AA);
       v[C3_c0] = c0;
@@ -736,6 +778,8 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C3_sqAB] = sqAB;
       v[C3_Ipd] = BB * rrp2 + AA * rrp1;
       v[C3_Imd] = BB * rrm2 + AA * rrm1;
+      v[C3_iIpd] = 1.0 / v[C3_Ipd];
+      v[C3_iImd] = 1.0 / v[C3_Imd];
     }
   } else {
     T rr1 = px.rr[0], rr2 = px.rr[1], rr3 = px.rr[2], rr4 = px.rr[3];
@@ -783,6 +827,9 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C2_drso] = ksqrt(Rpot_o);
       v[C2_ro3] = r_o - rr3;
       v[C2_h12] = 0.5 * (rr1 * rr4 + rr2 * rr3);
+      v[C2_irp3] = 1.0 / rrp3;
+      v[C2_irm3] = 1.0 / rrm3;
+      v[C2_Ho] = v[C2_drso] / v[C2_ro3];
     }
   }
 }
@@ -800,7 +847,27 @@ template <class T> struct RState {
   T s, c, d;
   T F, EmF;
   T Pi[3];
+  T lg[3];  // cases 3 and 4: the logs of the three R1 / S1 blocks
 };
+
+// The block logs of cases 3 and 4 at one Jacobi state, as log_ratio(a, b) =
+// ln|(b + a)/(b - a)|: R1 a = sn, b = p1 dn; S1 a = p2 dn, b = 1.
+template <class T, int CASE>
+KG_INLINE void block_log_args(const RadialConsts<T>& rc, int i, const T& sn, const T& dn, T& a, T& b) {
+  const T* v = rc.v;
+  if (CASE == 3) {
+    a = sn;
+    b = v[C3_p0 + i] * dn;
+  } else {
+    a = v[C4_q0 + i] * dn;
+    b = T(1.0);
+  }
+}
+
+template <class T> KG_INLINE T log_ratio(const T& a, const T& b) {
+  const T arg = (kfabs(val(a)) < kfabs(val(b))) ? 2.0 * a / (b - a) : 2.0 * b / (a - b);
+  return klog1p(arg);
+}
 
 template <class T> KG_INLINE T radial_arg(const RadialConsts<T>& rc, const T& tau) {
   return rc.v[R_U0] - rc.v[R_Xs] * tau;
@@ -855,6 +922,16 @@ KG_INLINE void radial_state(const RadialConsts<T>& rc, const JacobiLadder<T>& la
   st.Pi[0] = fam.Pi[0];
   st.Pi[1] = fam.Pi[1];
   st.Pi[2] = fam.Pi[2];
+  const int rcase = (CASE >= 0) ? CASE : rc.rcase;
+  if (rcase == 3 || rcase == 4) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      T a, b;
+      if (rcase == 3) block_log_args<T, 3>(rc, i, st.s, st.d, a, b);
+      else block_log_args<T, 4>(rc, i, st.s, st.d, a, b);
+      st.lg[i] = log_ratio(a, b);
+    }
+  }
 }
 
 // r(tau) and, with do_phi_t, the building blocks I_+, I_-, I_1, I_2
@@ -877,10 +954,10 @@ KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const
     if (!do_phi_t) return o;
     const T& F = st.F;
     T E = F + st.EmF;
-    T S1a0 = S1_block(g0, st.Pi[0], F, dn, k);
+    T S1a0 = S1_from(g0, st.Pi[0], F, v[C4_q0], st.lg[0], v[C4_l0], v[C4_i0]);
     T S2a0 = S2_block(g0, S1a0, F, E, sn, cn, dn, k);
-    T S1ap = S1_block(v[C4_gp], st.Pi[1], F, dn, k);
-    T S1am = S1_block(v[C4_gm], st.Pi[2], F, dn, k);
+    T S1ap = S1_from(v[C4_gp], st.Pi[1], F, v[C4_qp], st.lg[1], v[C4_lp], v[C4_ip]);
+    T S1am = S1_from(v[C4_gm], st.Pi[2], F, v[C4_qm], st.lg[2], v[C4_lm], v[C4_im]);
     T Pi1 = v[C4_P1f] * (S1a0 - v[C4_S1b0]);
     T Pi2 = v[C4_P2f] * (S2a0 - v[C4_S2b0]);
     T Pip = v[C4_Ppf] * (S1ap - v[C4_S1bp]);
@@ -902,10 +979,10 @@ KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const
     const bool hasm = val(v[C3_hasm]) != 0.0;
     const T& F = st.F;
     T E = F + st.EmF;
-    T R1a0 = R1_block(v[C3_al0], st.Pi[0], sn, dn, k);
+    T R1a0 = R1_from(v[C3_al0], st.Pi[0], v[C3_p0], st.lg[0], v[C3_i0]);
     T R2a0 = R2_block(v[C3_al0], v[C3_nn0], R1a0, F, E, sn, cn, dn, k);
-    T R1ap = R1_block(v[C3_alp], st.Pi[1], sn, dn, k);
-    T R1am = hasm ? R1_block(v[C3_alm], st.Pi[2], sn, dn, k) : T(0.0);
+    T R1ap = R1_from(v[C3_alp], st.Pi[1], v[C3_pp], st.lg[1], v[C3_ip]);
+    T R1am = hasm ? R1_from(v[C3_alm], st.Pi[2], v[C3_pm], st.lg[2], v[C3_im]) : T(0.0);
     T Pi1 = v[C3_c0] * (R1a0 - v[C3_R1b0]);
     T Pi2 = (v[C3_c0] * v[C3_c0]) * (R2a0 - v[C3_R2b0]);
     T Pip = v[C3_cp] * (R1ap - v[C3_R1bp]);
@@ -913,8 +990,8 @@ KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const
     const T& pr = v[C3_pr];
     o.I1 = pr * (-tau) + Pi1;
     o.I2 = (pr * pr) * (-tau) + 2.0 * pr * Pi1 + v[C3_sqAB] * Pi2;
-    o.Ip = -((BB + AA) * (-tau) + Pip) / v[C3_Ipd];
-    o.Im = -((BB + AA) * (-tau) + Pim) / v[C3_Imd];
+    o.Ip = -((BB + AA) * (-tau) + Pip) * v[C3_iIpd];
+    o.Im = -((BB + AA) * (-tau) + Pim) * v[C3_iImd];
   } else {
     const T& rr3 = v[C2_rr3];
     const T& rr4 = v[C2_rr4];
@@ -926,8 +1003,9 @@ KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const
     if (!do_phi_t) return o;
     const T& rr43 = v[C2_rr43];
     T dsn2 = 2.0 * sn * cn * dn * v[C2_dX];
-    T drs = -(rr31 * rr43 * rr41 * dsn2 / (den * den));
-    T H = drs / (o.r - rr3) - v[C2_drso] / v[C2_ro3];
+    // dr/dX / (r - r3) = -rr31 rr43 rr41 dsn2 / (den^2 (r - r3)), and
+    // den (r - r3) = rr31 rr43, so H needs no further division
+    T H = -(rr41 * dsn2) / den - v[C2_Ho];
     T Ea = st.F + st.EmF;
     T E = v[C2_sq] * (Ea - v[C2_Eb]);
     T Pi1 = v[C2_c] * (st.Pi[0] - v[C2_Pb1]);
@@ -935,8 +1013,8 @@ KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const
     T Pim = v[C2_fm] * (st.Pi[2] - v[C2_Pbm]);
     o.I1 = rr3 * (-tau) + rr43 * Pi1;
     o.I2 = H - v[C2_h12] * (-tau) - E;
-    o.Ip = tau / v[C2_rrp3] - Pip;
-    o.Im = tau / v[C2_rrm3] - Pim;
+    o.Ip = tau * v[C2_irp3] - Pip;
+    o.Im = tau * v[C2_irm3] - Pim;
   }
   return o;
 }

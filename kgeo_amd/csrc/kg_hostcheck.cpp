@@ -209,12 +209,12 @@ int kgh_walk_step(const double* at_u, const double* at_v, double u, double v, do
   const double nn[1] = {n};
   walk_init(w, m, nn);
   a.s = at_u[0]; a.c = at_u[1]; a.d = at_u[2]; a.D = at_u[3];
-  a.P[0] = walked_pi(w, 0, at_u[4], u, a.s, a.c, a.d);
+  walk_enter(w, 0, at_u[4], u, a.s, a.c, a.d, a);
   w.b.s = at_v[0]; w.b.c = at_v[1]; w.b.d = at_v[2]; w.b.D = at_v[3];
-  w.b.P[0] = walked_pi(w, 0, at_v[4], v, w.b.s, w.b.c, w.b.d);
+  walk_enter(w, 0, at_v[4], v, w.b.s, w.b.c, w.b.d, w.b);
   const bool ok = walk_step<double, 1, true>(a, w, r);
   out[0] = r.s; out[1] = r.c; out[2] = r.d; out[3] = r.D;
-  out[4] = original_pi(w, 0, r.P[0], u + v, r.s, r.c, r.d);
+  out[4] = walk_pi(w, 0, r, u + v);
   return ok ? 1 : 0;
 }
 

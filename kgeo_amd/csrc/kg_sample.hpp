@@ -125,7 +125,7 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<T>& tc, con
     w.b.c = sb.c;
     w.b.d = sb.d;
     w.b.D = DO_PHI_T ? sb.EmF / m : T(0.0);
-    w.b.P[0] = DO_PHI_T ? walked_pi(w, 0, sb.Pi, du, sb.s, sb.c, sb.d) : T(0.0);
+    if (DO_PHI_T) walk_enter(w, 0, sb.Pi, du, sb.s, sb.c, sb.d, w.b);
   }
   WalkState<T, 1> st;
   bool bad = false;
@@ -144,7 +144,7 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<T>& tc, con
         ts.d = st.d;
         ts.F = snarg;
         ts.EmF = m * st.D;
-        ts.Pi = DO_PHI_T ? original_pi(w, 0, st.P[0], snarg, st.s, st.c, st.d) : T(0.0);
+        ts.Pi = DO_PHI_T ? walk_pi(w, 0, st, snarg) : T(0.0);
       } else {
         direct = true;
       }
@@ -155,7 +155,7 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<T>& tc, con
       st.c = ts.c;
       st.d = ts.d;
       st.D = DO_PHI_T ? ts.EmF / m : T(0.0);
-      st.P[0] = DO_PHI_T ? walked_pi(w, 0, ts.Pi, snarg, ts.s, ts.c, ts.d) : T(0.0);
+      if (DO_PHI_T) walk_enter(w, 0, ts.Pi, snarg, ts.s, ts.c, ts.d, st);
     }
     T th = T(0.0), Gph = T(0.0), Gt = T(0.0);
     theta_out<T, KIND>(tc, snarg, ts, DO_PHI_T, th, Gph, Gt);
@@ -183,9 +183,12 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
     w.b.d = sb.d;
     w.b.D = DO_PHI_T ? sb.EmF / m : T(0.0);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) w.b.P[q] = DO_PHI_T ? walked_pi(w, q, sb.Pi[q], dX, sb.s, sb.c, sb.d) : T(0.0);
+    for (int q = 0; q < 3; ++q)
+      if (DO_PHI_T) walk_enter(w, q, sb.Pi[q], dX, sb.s, sb.c, sb.d, w.b);
   }
+  constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
   WalkState<T, 3> st;
+  T Lb[3];  // cases 3, 4: walked logs of the R1 / S1 blocks
   bool bad = false;
   for (int64_t j = j0; j < j1; ++j) {
     const int64_t idx = j * o.npix + i;
@@ -197,6 +200,16 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
     if (!direct) {
       WalkState<T, 3> nx;
       if (walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
+        if (kBlocks) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            T a0, b0, a1, b1;
+            block_log_args<T, CASE>(rc, q, st.s, st.d, a0, b0);
+            block_log_args<T, CASE>(rc, q, nx.s, nx.d, a1, b1);
+            Lb[q] = walk_log(Lb[q], a0, b0, a1, b1);
+            rs.lg[q] = Lb[q];
+          }
+        }
         st = nx;
         rs.s = st.s;
         rs.c = st.c;
@@ -204,7 +217,7 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
         rs.F = X;
         rs.EmF = m * st.D;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) rs.Pi[q] = DO_PHI_T ? original_pi(w, q, st.P[q], X, st.s, st.c, st.d) : T(0.0);
+        for (int q = 0; q < 3; ++q) rs.Pi[q] = DO_PHI_T ? walk_pi(w, q, st, X) : T(0.0);
       } else {
         direct = true;
       }
@@ -216,7 +229,10 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
       st.d = rs.d;
       st.D = DO_PHI_T ? rs.EmF / m : T(0.0);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) st.P[q] = DO_PHI_T ? walked_pi(w, q, rs.Pi[q], X, rs.s, rs.c, rs.d) : T(0.0);
+      for (int q = 0; q < 3; ++q) {
+        if (DO_PHI_T) walk_enter(w, q, rs.Pi[q], X, rs.s, rs.c, rs.d, st);
+        if (kBlocks) Lb[q] = rs.lg[q];
+      }
     }
     RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
     bad |= radial_store<T, DO_PHI_T>(o, cmb, idx, tau, ro);

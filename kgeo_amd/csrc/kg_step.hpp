@@ -63,6 +63,7 @@ template <class T, int NP> struct WalkState {
   T s, c, d;
   T D;
   T P[NP];
+  T Lg[NP];  // ln|(1 + x)/(1 - x)| of a transformed integral (below)
 };
 
 // Per-pixel constants of a walk: the state at the step, the parameter and,
@@ -72,24 +73,26 @@ template <class T, int NP> struct WalkStep {
   T m;
   T n[NP], w2[NP];  // the walked characteristic and n (m - n)(1 - n)
   T pt[NP];         // sqrt((n - 1)(1 - m/n)) of a transformed integral, else 0
+  T ipt2[NP];       // 1 / (2 pt)
   bool tr[NP];
 };
 
 // Characteristics n > 1 (principal value) are walked through
-//   Pi(n; am u | m) = -Pi(m/n; am u | m) + u + L(u),
-//   L(u) = ln|(1 + x)/(1 - x)| / (2 pt),  x = pt sn u / (cn u dn u),
+//   Pi(n; am u | m) = -Pi(m/n; am u | m) + u + Lg(u) / (2 pt),
+//   Lg(u) = ln|(1 + x)/(1 - x)|,  x = pt sn u / (cn u dn u),
 //   pt = sqrt((n - 1)(1 - m/n))                           (DLMF 19.7.9)
-// whose right-hand integral has 0 < m/n < 1 and no pole: the log carries the
-// singularity and is evaluated per sample.  Used for 0 <= m < 1, n > 1 and
-// pt >= kTransformMinPt (the log term amplifies rounding by ~1/pt); anywhere
+// whose right-hand integral has 0 < m/n < 1 and no pole: Lg carries the
+// singularity.  Lg walks too: for any real x0, x1
+//   Lg(x1) - Lg(x0) = ln|(1 + y)/(1 - y)| = 2 y R_C(1, 1 - y^2),
+//   y = (x1 - x0)/(1 - x0 x1),
+// and with x = pt sn/(cn dn) the numerator is, by the subtraction theorem
+// sn(a - b) (1 - m sn^2 a sn^2 b) = sn a cn b dn b - sn b cn a dn a,
+//   y = pt sn v (1 - m sn^2(u+v) sn^2 u) / (cn(u+v) dn(u+v) cn u dn u - pt^2 sn(u+v) sn u),
+// free of cancellation.  Where |y^2| > kStepZ2 (next to the pole) the lane
+// takes Lg from its closed form instead.  Used for 0 <= m < 1, n > 1 and
+// pt >= kTransformMinPt (1/pt amplifies rounding in Lg / (2 pt)); anywhere
 // else n is walked as it is.
 constexpr double kTransformMinPt = 0.05;
-
-template <class T> KG_INLINE T pv_log_term(const T& pt, const T& sn, const T& cn, const T& dn) {
-  const T x = pt * sn / (cn * dn);
-  const T l = (kfabs(val(x)) < 1.0) ? klog1p(2.0 * x / (1.0 - x)) : klog1p(2.0 / (x - 1.0));
-  return l / (2.0 * pt);
-}
 
 template <class T, int NP>
 KG_INLINE void walk_init(WalkStep<T, NP>& w, const T& m, const T (&n)[NP]) {
@@ -101,22 +104,41 @@ KG_INLINE void walk_init(WalkStep<T, NP>& w, const T& m, const T (&n)[NP]) {
     w.tr[i] = cand && (val(pt) >= kTransformMinPt);
     const T nw = w.tr[i] ? m / n[i] : n[i];
     w.pt[i] = w.tr[i] ? pt : T(0.0);
+    w.ipt2[i] = w.tr[i] ? 0.5 / pt : T(0.0);
     w.n[i] = nw;
     w.w2[i] = nw * (m - nw) * (1.0 - nw);
   }
 }
 
-// Pi(n; am u) -> the walked integral at the same argument, and back.
+// Direct entry into the walk at argument u: from Pi(n; am u), set the walked
+// integral (and Lg) of slot i.
 template <class T, int NP>
-KG_INLINE T walked_pi(const WalkStep<T, NP>& w, int i, const T& Pn, const T& u, const T& sn, const T& cn,
-                      const T& dn) {
-  return w.tr[i] ? -Pn + u + pv_log_term(w.pt[i], sn, cn, dn) : Pn;
+KG_INLINE void walk_enter(const WalkStep<T, NP>& w, int i, const T& Pn, const T& u, const T& sn, const T& cn,
+                          const T& dn, WalkState<T, NP>& st) {
+  if (w.tr[i]) {
+    st.Lg[i] = log_ratio(w.pt[i] * sn, cn * dn);
+    st.P[i] = -Pn + u + st.Lg[i] * w.ipt2[i];
+  } else {
+    st.Lg[i] = T(0.0);
+    st.P[i] = Pn;
+  }
 }
 
+// Pi(n; am u) of slot i from the walked state at argument u.
 template <class T, int NP>
-KG_INLINE T original_pi(const WalkStep<T, NP>& w, int i, const T& Pw, const T& u, const T& sn, const T& cn,
-                        const T& dn) {
-  return w.tr[i] ? -Pw + u + pv_log_term(w.pt[i], sn, cn, dn) : Pw;
+KG_INLINE T walk_pi(const WalkStep<T, NP>& w, int i, const WalkState<T, NP>& st, const T& u) {
+  return w.tr[i] ? -st.P[i] + u + st.Lg[i] * w.ipt2[i] : st.P[i];
+}
+
+// One step of a walked log ratio Lg = ln|(b + a)/(b - a)| from (a0, b0) to
+// (a1, b1): Lg1 = Lg0 + ln|(1 + y)/(1 - y)|, y = (a1 b0 - a0 b1)/(b0 b1 - a0 a1),
+// by the series while |y^2| <= kStepZ2, else from its closed form.
+template <class T>
+KG_INLINE T walk_log(const T& Lg0, const T& a0, const T& b0, const T& a1, const T& b1) {
+  const T y = (a1 * b0 - a0 * b1) / (b0 * b1 - a0 * a1);
+  const T y2 = y * y;
+  if (kfabs(val(y2)) <= kStepZ2) return Lg0 + 2.0 * y * rc_one_series(y2);
+  return log_ratio(a1, b1);
 }
 
 // One step a -> a + du.  Returns false (leaving `out` unspecified) when the
@@ -137,20 +159,57 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
   const T q = ss * S;
   out.D = a.D + b.D - q;
   const T cd = ss * C * Dn;
+  // the NP values of p and one reciprocal of their product
+  T pv[NP], r1v[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    r1v[i] = 1.0 - w.n[i] * (S * S);
+    pv[i] = r1v[i] + w.n[i] * cd;
+  }
+  T ipv[NP];
+  if (NP == 1) {
+    ipv[0] = 1.0 / pv[0];
+  } else {
+    T prod = pv[0];
+#pragma unroll
+    for (int i = 1; i < NP; ++i) prod = prod * pv[i];
+    const T iprod = 1.0 / prod;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      T o = iprod;
+#pragma unroll
+      for (int k2 = 0; k2 < NP; ++k2)
+        if (k2 != i) o = o * pv[k2];
+      ipv[i] = o;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     // 1 - n sn^2 at both ends and p must share a sign: no pole of the
     // integrand (n > 1: principal value) lies inside the step
     const T r0 = 1.0 - w.n[i] * (a.s * a.s);
-    const T r1 = 1.0 - w.n[i] * (S * S);
-    const T p = r1 + w.n[i] * cd;
-    const T ip = 1.0 / p;
+    const T& r1 = r1v[i];
+    const T& p = pv[i];
+    const T& ip = ipv[i];
     const T nq = w.n[i] * q * ip;
     const T z2 = w.w2[i] * (q * ip) * (q * ip);
     ok = ok && ((val(r0) > 0.0) == (val(r1) > 0.0)) && ((val(p) > 0.0) == (val(r1) > 0.0)) &&
          (kfabs(val(z2)) <= kStepZ2);
     out.P[i] = a.P[i] + b.P[i] + nq * rc_one_series(z2);
     ok = ok && kfinite(val(out.P[i]));
+    if (w.tr[i]) {
+      const T pt = w.pt[i];
+      const T y = pt * b.s * (1.0 - w.m * (S * S) * (a.s * a.s)) / (C * Dn * (a.c * a.d) - pt * pt * S * a.s);
+      const T y2 = y * y;
+      if (kfabs(val(y2)) <= kStepZ2) {
+        out.Lg[i] = a.Lg[i] + 2.0 * y * rc_one_series(y2);
+      } else {
+        out.Lg[i] = log_ratio(pt * S, C * Dn);
+      }
+      ok = ok && kfinite(val(out.Lg[i]));
+    } else {
+      out.Lg[i] = T(0.0);
+    }
   }
   return ok && kfinite(val(out.D));
 }
