@@ -13,7 +13,8 @@
 //
 // On the linspace grid the loops walk: each sample's state follows from the
 // previous one by the addition theorems (kg_step.hpp), with a direct
-// evaluation at the chunk's first sample, at every multiple of kAnchorEvery,
+// evaluation at the chunk's first sample, at every multiple of the anchor
+// interval (kg_step.hpp),
 // at the last grid sample (tau = stop exactly) and wherever a step is not
 // well conditioned.  With caller-given Mino times (tau_in) every sample is direct.
 // Host-compilable (KG_INLINE), so the CPU test build runs the same loops.
@@ -134,7 +135,7 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<T>& tc, con
     const double tau = mino_tau(j, o.ngeo, stop, false);
     const T snarg = theta_arg(tc, T(tau));
     ThState<T> ts;
-    bool direct = (j == j0) || (j % kAnchorEvery == 0) || (j == o.ngeo - 1);
+    bool direct = (j == j0) || (j % kAnchorEveryLong == 0) || (j == o.ngeo - 1);
     if (!direct) {
       WalkState<T, 1> nx;
       if (walk_step<T, 1, DO_PHI_T>(st, w, nx)) {
@@ -187,6 +188,9 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
       if (DO_PHI_T) walk_enter(w, q, sb.Pi[q], dX, sb.s, sb.c, sb.d, w.b);
   }
   constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
+  // case 3 and 4 outputs amplify the walk's rounding (1/(1 - al^2) in the R1
+  // blocks), so they restart from a direct evaluation twice as often
+  constexpr int kAnchor = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
   WalkState<T, 3> st;
   T Lb[3];  // cases 3, 4: walked logs of the R1 / S1 blocks
   bool bad = false;
@@ -196,7 +200,7 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
     const T dv = rc.v[R_Xs] * tau;
     const T X = rc.v[R_U0] - dv;
     RState<T> rs;
-    bool direct = (j == j0) || (j % kAnchorEvery == 0) || (j == o.ngeo - 1);
+    bool direct = (j == j0) || (j % kAnchor == 0) || (j == o.ngeo - 1);
     if (!direct) {
       WalkState<T, 3> nx;
       if (walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
@@ -241,14 +245,14 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
 }
 
 // First sample of chunk c of `chunks` over ngeo samples.  Chunk starts are
-// multiples of kAnchorEvery, so the walk's direct evaluations fall on the
-// same samples (multiples of kAnchorEvery, the last sample, and the
-// ill-conditioned steps) for every launch shape: results are bit-identical
-// whatever the chunking.
+// multiples of kAnchorEveryLong (itself a multiple of kAnchorEvery), so the
+// walk's direct evaluations fall on the same samples (the anchor multiples,
+// the last sample and the ill-conditioned steps) for every launch shape:
+// results are bit-identical whatever the chunking.
 KG_INLINE int64_t chunk_start(int64_t ngeo, int64_t c, int64_t chunks) {
   if (c <= 0) return 0;
   if (c >= chunks) return ngeo;
-  return ((ngeo * c) / chunks) / kAnchorEvery * kAnchorEvery;
+  return ((ngeo * c) / chunks) / kAnchorEveryLong * kAnchorEveryLong;
 }
 
 // Walking needs the linspace grid with at least one interior step: not for

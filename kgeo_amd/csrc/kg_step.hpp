@@ -24,8 +24,9 @@
 // (where the 15-term series of R_C(1, 1 - z^2) is exact to rounding) and
 // every stepped value finite.  Otherwise the sample is evaluated directly
 // and the walk continues from there.  The loops also restart from a direct
-// evaluation every kAnchorEvery samples, which bounds the accumulated
-// rounding to a few tens of ulps.
+// evaluation every kAnchorEveryLong samples (theta, radial case 2) or
+// kAnchorEvery samples (radial cases 3 and 4, whose R1/S1 blocks amplify
+// rounding), which bounds the accumulated rounding to a few tens of ulps.
 
 #pragma once
 
@@ -34,7 +35,8 @@
 namespace kg {
 
 constexpr double kStepZ2 = 0.04;
-constexpr int kAnchorEvery = 16;
+constexpr int kAnchorEvery = 16;       // radial cases 3 and 4
+constexpr int kAnchorEveryLong = 32;   // theta and radial case 2
 
 // R_C(1, 1 - z2) = sum_k z2^k / (2k + 1) for |z2| <= kStepZ2
 // (0.04^15 / 31 < 1e-22: the truncation is below rounding)

@@ -263,8 +263,9 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
     B.slot_off = slots;
     B.out_off = outo;
     B.item_off = items;
-    // items hold whole anchor intervals (chunk starts are multiples of kAnchorEvery)
-    const int64_t spi = (static_cast<int64_t>(tn.samples_per_item) + kAnchorEvery - 1) / kAnchorEvery * kAnchorEvery;
+    // items hold whole anchor intervals (chunk starts are multiples of kAnchorEveryLong)
+    const int64_t spi =
+        (static_cast<int64_t>(tn.samples_per_item) + kAnchorEveryLong - 1) / kAnchorEveryLong * kAnchorEveryLong;
     const int64_t ch = (p.ngeo + spi - 1) / spi;
     B.chunks = static_cast<int32_t>(ch);
     B.do_phi_t = (p.flags & KGEO_DO_PHI_T) ? 1 : 0;
