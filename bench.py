@@ -7,12 +7,16 @@ r_o = 1e4, ngeo = 500 Mino-time samples per ray: per-pixel setup (kgeo_plan)
 plus all 131,072,000 samples with r, theta, phi, t written to HBM
 (kgeo_sample).  Inputs are resident in HBM before the timed region.
 
-Prints ONE JSON line with the metric, the roofline of the dominant kernel
-(kg_samples, FP64 vector ALU bound; timed with HIP events on the launch
-stream) and the CPU baseline (the oracle, i.e. the reference algorithm in
-NumPy/SciPy, on a bounded seeded sample of the same workload on this host).
+Prints ONE JSON line with the metric, the roofline of the sample kernels
+(kg_theta + kg_radial, FP64 vector ALU; timed with HIP events on the launch
+stream; algorithmic FLOPs op-counted on the kernels' own loops) and the CPU
+baseline (the oracle, i.e. the reference algorithm in NumPy/SciPy, on a
+bounded seeded sample of the same workload on this host).
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+  python bench.py --mode batch        256-image survey, one persistent graph launch
+  python bench.py --mode equatorial   equatorial images/s and gradient images/s
+  python bench.py --mode copy         plain 1 GiB copy (counter calibration)
 """
 
 import argparse
@@ -124,13 +128,17 @@ def cpu_baseline(A, B, th, px_per_core=768, seed=1):
             "scipy": scipy.__version__}
 
 
-def read_traffic():
-    """HBM bytes per kg_samples launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "kg_samples_512x512_ngeo500_pmc.txt")
-    if not os.path.exists(path):
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "kg_theta_kg_radial_512x512_ngeo500_pmc.txt")
+
+
+def read_profile_value(key):
+    """A figure from the committed rocprofv3 counter summary of kg_theta +
+    kg_radial at this workload (FETCH_SIZE + WRITE_SIZE scaled by a plain
+    copy's reading; hardware FP64 FLOP count), or None."""
+    if not os.path.exists(PMC_SUMMARY):
         return None
-    for ln in open(path):
-        if ln.startswith("traffic_bytes_per_launch:"):
+    for ln in open(PMC_SUMMARY):
+        if ln.startswith(key + ":"):
             return float(ln.split(":", 1)[1])
     return None
 
@@ -381,10 +389,12 @@ def main():
     bytes_per_launch = 32.0 * samples
     hbm_gbs = bytes_per_launch / (samp_ms * 1e-3) / 1e9
     roofline = {"bound": "valu-fp64", "kernel": "kg_theta + kg_radial", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
+                "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS,
+                "traffic": read_profile_value("traffic_bytes_per_launch"),
                 "flops_per_sample": flops_s, "algorithmic_bytes_per_launch": bytes_per_launch,
                 "kernel_ms": samp_ms, "hbm_write_GBs": hbm_gbs, "hbm_frac": hbm_gbs / HBM_PEAK_GBS,
                 "setup_kernel_ms": plan_ms, "setup_flops_per_pixel": flops_setup,
+                "hardware_fp64_flops_per_launch": read_profile_value("hardware_fp64_flops_per_launch"),
                 "ops_per_sample": {k: round(v, 3) for k, v in ops.items()}}
 
     cpu = None
