@@ -129,6 +129,7 @@ template <int N> KG_INLINE Dual<N> kexp(const Dual<N>& x) {
   return chain(x, e, e);
 }
 template <int N> KG_INLINE Dual<N> kfabs(const Dual<N>& x) { return (x.v < 0.0) ? -x : x; }
+template <int N> KG_INLINE Dual<N> frcp(const Dual<N>& x) { return 1.0 / x; }
 template <int N> KG_INLINE Dual<N> kpow(const Dual<N>& x, double y) {
   const double p = ::pow(x.v, y);
   return chain(x, p, y * ::pow(x.v, y - 1.0));

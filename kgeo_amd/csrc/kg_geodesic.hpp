@@ -527,13 +527,14 @@ enum RSlot {
   C3_p0, C3_pp, C3_pm,  // p1 of the three R1 blocks
   C3_i0, C3_ip, C3_im,  // 1/(1 - al^2) of the three R1 blocks
   C3_iIpd, C3_iImd,     // 1/Ipd, 1/Imd
+  C3_r2a, C3_r2b, C3_r2c,  // R2 block: al^2/den, 1/(al^2 - 1), (2k - n)/den
   // case 4 (kerr_raytracing_ana.py:404-471)
   C4_a2 = 8, C4_b1, C4_g0, C4_gp, C4_gm, C4_n0, C4_np, C4_nm, C4_Pc0, C4_Pcp, C4_Pcm,
   C4_S1b0, C4_S2b0, C4_S1bp, C4_S1bm, C4_P1f, C4_P2f, C4_Ppf, C4_Pmf, C4_aa1, C4_Ipf, C4_Imf,
   C4_q0, C4_qp, C4_qm,  // p2 of the three S1 blocks
   C4_l0, C4_lp, C4_lm,  // ln|(1 - p2)/(1 + p2)| of the three S1 blocks
   C4_i0, C4_ip, C4_im,  // 1/(1 + al^2) of the three S1 blocks
-  R_NSLOT = 41
+  R_NSLOT = 44
 };
 
 template <class T> struct RadialConsts {
@@ -780,6 +781,13 @@ KG_INLINE void radial_consts(const RayScalars<T>& c, const PixelSetup<T>& px, bo
       v[C3_Imd] = BB * rrm2 + AA * rrm1;
       v[C3_iIpd] = 1.0 / v[C3_Ipd];
       v[C3_iImd] = 1.0 / v[C3_Imd];
+      {
+        const T al2 = al0 * al0;
+        const T den = k3 + (1.0 - k3) * al2;
+        v[C3_r2a] = al2 / den;
+        v[C3_r2b] = 1.0 / (al2 - 1.0);
+        v[C3_r2c] = (2.0 * k3 - nn0) / den;
+      }
     }
   } else {
     T rr1 = px.rr[0], rr2 = px.rr[1], rr3 = px.rr[2], rr4 = px.rr[3];
@@ -980,7 +988,9 @@ KG_INLINE RadialOut<T> radial_out(const RadialConsts<T>& rc, const T& tau, const
     const T& F = st.F;
     T E = F + st.EmF;
     T R1a0 = R1_from(v[C3_al0], st.Pi[0], v[C3_p0], st.lg[0], v[C3_i0]);
-    T R2a0 = R2_block(v[C3_al0], v[C3_nn0], R1a0, F, E, sn, cn, dn, k);
+    // the R2 block of R2_block with its per-pixel factors precomputed
+    const T& al0 = v[C3_al0];
+    T R2a0 = (F - v[C3_r2a] * (E - al0 * sn * dn / (1.0 + al0 * cn))) * v[C3_r2b] + v[C3_r2c] * R1a0;
     T R1ap = R1_from(v[C3_alp], st.Pi[1], v[C3_pp], st.lg[1], v[C3_ip]);
     T R1am = hasm ? R1_from(v[C3_alm], st.Pi[2], v[C3_pm], st.lg[2], v[C3_im]) : T(0.0);
     T Pi1 = v[C3_c0] * (R1a0 - v[C3_R1b0]);

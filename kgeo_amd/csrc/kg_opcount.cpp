@@ -53,6 +53,7 @@ inline CD klog(const CD& x) { ++g_ops[LOG]; return CD(std::log(x.v)); }
 inline CD klog1p(const CD& x) { ++g_ops[LOG1P]; return CD(std::log1p(x.v)); }
 inline CD kexp(const CD& x) { ++g_ops[EXP]; return CD(std::exp(x.v)); }
 inline CD kfabs(const CD& x) { return CD(std::fabs(x.v)); }
+inline CD frcp(const CD& x) { ++g_ops[DIV]; return CD(1.0 / x.v); }
 inline CD kpow(const CD& x, double y) { ++g_ops[POW]; return CD(std::pow(x.v, y)); }
 
 }  // namespace kgcount

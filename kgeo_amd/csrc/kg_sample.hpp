@@ -205,12 +205,17 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
       WalkState<T, 3> nx;
       if (walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
         if (kBlocks) {
+          T a0[3], b0[3], a1[3], b1[3], den[3], iden[3];
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
-            T a0, b0, a1, b1;
-            block_log_args<T, CASE>(rc, q, st.s, st.d, a0, b0);
-            block_log_args<T, CASE>(rc, q, nx.s, nx.d, a1, b1);
-            Lb[q] = walk_log(Lb[q], a0, b0, a1, b1);
+            block_log_args<T, CASE>(rc, q, st.s, st.d, a0[q], b0[q]);
+            block_log_args<T, CASE>(rc, q, nx.s, nx.d, a1[q], b1[q]);
+            den[q] = walk_log_den(a0[q], b0[q], a1[q], b1[q]);
+          }
+          shared_reciprocals<T, 3>(den, iden);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            Lb[q] = walk_log(Lb[q], a0[q], b0[q], a1[q], b1[q], iden[q]);
             rs.lg[q] = Lb[q];
           }
         }

@@ -132,15 +132,40 @@ KG_INLINE T walk_pi(const WalkStep<T, NP>& w, int i, const WalkState<T, NP>& st,
   return w.tr[i] ? -st.P[i] + u + st.Lg[i] * w.ipt2[i] : st.P[i];
 }
 
+// 1/x_i for NP values with one reciprocal of their product.
+template <class T, int NP> KG_INLINE void shared_reciprocals(const T (&x)[NP], T (&ix)[NP]) {
+  if (NP == 1) {
+    ix[0] = frcp(x[0]);
+    return;
+  }
+  T prod = x[0];
+#pragma unroll
+  for (int i = 1; i < NP; ++i) prod = prod * x[i];
+  const T ip = frcp(prod);
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    T o = ip;
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+      if (k != i) o = o * x[k];
+    ix[i] = o;
+  }
+}
+
 // One step of a walked log ratio Lg = ln|(b + a)/(b - a)| from (a0, b0) to
 // (a1, b1): Lg1 = Lg0 + ln|(1 + y)/(1 - y)|, y = (a1 b0 - a0 b1)/(b0 b1 - a0 a1),
-// by the series while |y^2| <= kStepZ2, else from its closed form.
+// by the series while |y^2| <= kStepZ2, else from its closed form.  The
+// caller passes 1/(b0 b1 - a0 a1) (walk_log_den), shared across integrals.
 template <class T>
-KG_INLINE T walk_log(const T& Lg0, const T& a0, const T& b0, const T& a1, const T& b1) {
-  const T y = (a1 * b0 - a0 * b1) / (b0 * b1 - a0 * a1);
+KG_INLINE T walk_log(const T& Lg0, const T& a0, const T& b0, const T& a1, const T& b1, const T& inv_den) {
+  const T y = (a1 * b0 - a0 * b1) * inv_den;
   const T y2 = y * y;
   if (kfabs(val(y2)) <= kStepZ2) return Lg0 + 2.0 * y * rc_one_series(y2);
   return log_ratio(a1, b1);
+}
+
+template <class T> KG_INLINE T walk_log_den(const T& a0, const T& b0, const T& a1, const T& b1) {
+  return b0 * b1 - a0 * a1;
 }
 
 // One step a -> a + du.  Returns false (leaving `out` unspecified) when the
@@ -149,7 +174,7 @@ template <class T, int NP, bool INTEGRALS>
 KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out) {
   const WalkState<T, NP>& b = w.b;
   const T ss = a.s * b.s;
-  const T inv = 1.0 / (1.0 - w.m * ss * ss);
+  const T inv = frcp(1.0 - w.m * ss * ss);
   const T S = (a.s * (b.c * b.d) + a.c * b.s * a.d) * inv;
   const T C = (a.c * b.c - ss * a.d * b.d) * inv;
   const T Dn = (a.d * b.d - w.m * ss * a.c * b.c) * inv;
@@ -169,22 +194,7 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
     pv[i] = r1v[i] + w.n[i] * cd;
   }
   T ipv[NP];
-  if (NP == 1) {
-    ipv[0] = 1.0 / pv[0];
-  } else {
-    T prod = pv[0];
-#pragma unroll
-    for (int i = 1; i < NP; ++i) prod = prod * pv[i];
-    const T iprod = 1.0 / prod;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      T o = iprod;
-#pragma unroll
-      for (int k2 = 0; k2 < NP; ++k2)
-        if (k2 != i) o = o * pv[k2];
-      ipv[i] = o;
-    }
-  }
+  shared_reciprocals<T, NP>(pv, ipv);
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     // 1 - n sn^2 at both ends and p must share a sign: no pole of the
@@ -199,14 +209,23 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
          (kfabs(val(z2)) <= kStepZ2);
     out.P[i] = a.P[i] + b.P[i] + nq * rc_one_series(z2);
     ok = ok && kfinite(val(out.P[i]));
+  }
+  // the transformed integrals' log terms, one shared reciprocal
+  T yd[NP], iyd[NP];
+  const T num = b.s * (1.0 - w.m * (S * S) * (a.s * a.s));
+  const T cdn = C * Dn * (a.c * a.d);
+#pragma unroll
+  for (int i = 0; i < NP; ++i) yd[i] = w.tr[i] ? cdn - (w.pt[i] * w.pt[i]) * (S * a.s) : T(1.0);
+  shared_reciprocals<T, NP>(yd, iyd);
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
     if (w.tr[i]) {
-      const T pt = w.pt[i];
-      const T y = pt * b.s * (1.0 - w.m * (S * S) * (a.s * a.s)) / (C * Dn * (a.c * a.d) - pt * pt * S * a.s);
+      const T y = w.pt[i] * num * iyd[i];
       const T y2 = y * y;
       if (kfabs(val(y2)) <= kStepZ2) {
         out.Lg[i] = a.Lg[i] + 2.0 * y * rc_one_series(y2);
       } else {
-        out.Lg[i] = log_ratio(pt * S, C * Dn);
+        out.Lg[i] = log_ratio(w.pt[i] * S, C * Dn);
       }
       ok = ok && kfinite(val(out.Lg[i]));
     } else {
