@@ -344,7 +344,7 @@ def main():
     nmax = torch.empty(npix, device=dev, dtype=torch.int32)
     tau_tot = torch.empty(npix, **f64)
     status = torch.empty(npix, device=dev, dtype=torch.int32)
-    wsb = _lib.workspace_bytes(npix)
+    wsb = _lib.workspace_bytes(npix, NGEO)
     ws = torch.empty(wsb, device=dev, dtype=torch.uint8)
     p = _lib.make_params(A_SPIN, R_O, th, npix, NGEO, 1.0 - 1e-10, True)
     stream = _stream_handle()

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define KGEO_ABI_VERSION 1
+#define KGEO_ABI_VERSION 2
 
 /* return codes */
 #define KGEO_OK 0
@@ -68,9 +68,16 @@ const char* kgeo_last_error(void);
 /* Fill the derived scalars of p from p->a and p->th_o (host-side, libm). */
 void kgeo_fill_params(kgeo_ray_params* p);
 
-/* Bytes of device workspace kgeo_raytrace_ana / kgeo_coords_at_tau need for
- * npix pixels (the per-pixel plan plus 256 bytes of flags). */
-size_t kgeo_workspace_bytes(int64_t npix);
+/* Bytes of device workspace for npix pixels sampled on a linspace grid of
+ * ngeo Mino times: 256 bytes of flags and counters, the per-pixel plan and,
+ * for ngeo > 2, the walk's record tables and fixup queue (about 4.6 kB per
+ * pixel at ngeo = 500).  Pass ngeo = 0 for kgeo_plan alone and for
+ * caller-given Mino times (kgeo_coords_at_tau, kgeo_sample with tau_in).
+ * Environment: KGEO_CHUNKS=<n> fixes the sample chunks per pixel;
+ * KGEO_ONE_PASS=1 evaluates the grid in one combined pass per coordinate
+ * instead of the anchor / walk / fixup passes (same results to rounding;
+ * for comparisons). */
+size_t kgeo_workspace_bytes(int64_t npix, int64_t ngeo);
 
 /*
  * Replaces kerr_raytracing_ana.raytrace_ana (kerr_raytracing_ana.py:46-122):
@@ -88,7 +95,7 @@ size_t kgeo_workspace_bytes(int64_t npix);
  *   nmax_eq          [npix] int32  optional: Geodesics.nmax_eq (utils.py:63-67)
  *   tau_tot          [npix]        optional: mino_total (utils.py:382-548)
  *   status           [npix] uint32 optional: KGEO_ST_* bits
- *   workspace        >= kgeo_workspace_bytes(npix) bytes, 256-byte aligned
+ *   workspace        >= kgeo_workspace_bytes(npix, ngeo) bytes, 256-byte aligned
  */
 int kgeo_raytrace_ana(const kgeo_ray_params* p, const double* alpha, const double* beta, double* tau, double* t,
                       double* r, double* th, double* ph, double* sig, int8_t* rclass, int8_t* thclass,
@@ -103,13 +110,14 @@ int kgeo_raytrace_ana(const kgeo_ray_params* p, const double* alpha, const doubl
  * :300-327, :404-434, :473-510, :554-584), written into `workspace`.
  * kgeo_sample is coords_at_tau (:97, :126-193) from that plan: tau_in
  * [ntau][npix] if non-NULL, else the linspace grid of p->ngeo samples
- * (then ntau must equal p->ngeo).  Outputs as in kgeo_raytrace_ana.
+ * (then ntau must equal p->ngeo, and the workspace must hold
+ * kgeo_workspace_bytes(npix, ntau)).  Outputs as in kgeo_raytrace_ana.
  */
 int kgeo_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, int8_t* rclass, int8_t* thclass,
               int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* workspace, size_t workspace_bytes,
               void* stream);
 int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, double* tau, double* t, double* r,
-                double* th, double* ph, double* sig, uint32_t* status, const void* workspace,
+                double* th, double* ph, double* sig, uint32_t* status, void* workspace,
                 size_t workspace_bytes, void* stream);
 
 /*

@@ -11,6 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
+# must equal KGEO_ABI_VERSION of include/kgeo.h
+ABI_VERSION = 2
 LIB_PATH = os.path.join(_HERE, "libkgeo_hip.so")
 
 KGEO_DO_PHI_T = 1
@@ -55,7 +57,7 @@ _PROTOS = {
     "kgeo_abi_version": (ctypes.c_int, []),
     "kgeo_last_error": (ctypes.c_char_p, []),
     "kgeo_fill_params": (None, [ctypes.POINTER(RayParams)]),
-    "kgeo_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64]),
+    "kgeo_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
     "kgeo_sample_chunks": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64]),
     "kgeo_raytrace_ana": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 14
                           + [ctypes.c_size_t, ctypes.c_void_p]),
@@ -108,7 +110,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.kgeo_abi_version() != 1:
+    if lib.kgeo_abi_version() != ABI_VERSION:
         raise KgeoError("libkgeo_hip.so ABI version mismatch")
     _lib = lib
     return lib
@@ -149,5 +151,7 @@ def make_params(a, r_o, th_o, npix, ngeo, maxtaufrac, do_phi_t=True):
     return p
 
 
-def workspace_bytes(npix):
-    return int(load().kgeo_workspace_bytes(int(npix)))
+def workspace_bytes(npix, ngeo=0):
+    """Device workspace bytes for npix pixels on a linspace grid of ngeo Mino
+    times (0: the plan alone, as caller-given Mino times need)."""
+    return int(load().kgeo_workspace_bytes(int(npix), int(ngeo)))

@@ -207,7 +207,7 @@ def run_sequential(images, arena, out_off, r_o=1.0e4, maxtaufrac=MAXTAUFRAC, do_
     lib = _lib.load()
     dev = _device()
     maxpix = max(len(im["alpha"]) for im in images)
-    wsb = _lib.workspace_bytes(maxpix)
+    wsb = _lib.workspace_bytes(maxpix, max(int(im["ngeo"]) for im in images))
     if workspace is None or workspace.numel() < wsb:
         workspace = torch.empty(wsb, device=dev, dtype=torch.uint8)
     stream = ctypes_void(torch.cuda.current_stream().cuda_stream)

@@ -1,0 +1,72 @@
+// The anchor / walk / fixup passes of the sample kernels (kg_sample.hpp,
+// kgeo_raytrace.hip) run on the CPU over all pixels, in the kernels' order:
+// theta anchors, theta walk, theta fixups, then the same for r.  Used by the
+// CPU test build (T = double) and the FLOP counter (T = its counting type).
+
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "kg_sample.hpp"
+
+namespace kg {
+
+// Fixup queue on the host: the kernels' 64-bit item (walked << 63 |
+// first sample << 32 | pixel).
+struct HostPush {
+  std::vector<uint64_t>* items;
+  void operator()(int64_t i, int64_t j, bool walked) const {
+    items->push_back((walked ? (1ull << 63) : 0ull) | (static_cast<uint64_t>(j) << 32) | static_cast<uint32_t>(i));
+  }
+};
+
+inline void unpack_host_item(uint64_t e, int64_t& i, int64_t& j, bool& walked) {
+  i = static_cast<int64_t>(e & 0xffffffffull);
+  j = static_cast<int64_t>((e >> 32) & 0x7fffffffull);
+  walked = (e >> 63) != 0ull;
+}
+
+// st [npix] accumulates the status bits; nfix[2] receives the theta and
+// radial queue lengths.
+template <bool DO_PHI_T, class T>
+void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, const SampleOut& o, int chunks,
+                  uint32_t* st, int64_t* nfix) {
+  const int64_t npix = o.npix, G = o.ngeo;
+  std::vector<double> trec(static_cast<size_t>(theta_records(G)) * kThFields * npix);
+  std::vector<double> rrec(static_cast<size_t>(radial_records(G)) * kRFields * npix);
+  const Records rth{trec.data(), npix}, rr{rrec.data(), npix};
+  std::vector<uint64_t> items;
+  HostPush push{&items};
+  for (int64_t i = 0; i < npix; ++i)
+    for (int64_t rec = 0; rec < theta_records(G); ++rec)
+      st[i] |= theta_anchor_pixel<DO_PHI_T, T>(plan, npix, i, gflags, o, i, rec, rth);
+  for (int64_t i = 0; i < npix; ++i)
+    for (int ch = 0; ch < chunks; ++ch)
+      st[i] |= theta_walk_pixel<DO_PHI_T, HostPush, T>(plan, npix, i, gflags, o, i, chunk_start(G, ch, chunks),
+                                                        chunk_start(G, ch + 1, chunks), rth, push);
+  nfix[0] = static_cast<int64_t>(items.size());
+  for (uint64_t e : items) {
+    int64_t i, j;
+    bool w;
+    unpack_host_item(e, i, j, w);
+    st[i] |= theta_fixup_pixel<DO_PHI_T, T>(plan, npix, i, gflags, o, i, j, w);
+  }
+  items.clear();
+  for (int64_t i = 0; i < npix; ++i)
+    for (int64_t rec = 0; rec < radial_records(G); ++rec)
+      st[i] |= radial_anchor_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, rec, rr);
+  for (int64_t i = 0; i < npix; ++i)
+    for (int ch = 0; ch < chunks; ++ch)
+      st[i] |= radial_walk_pixel<DO_PHI_T, HostPush, T>(c, plan, npix, i, gflags, o, i, chunk_start(G, ch, chunks),
+                                                         chunk_start(G, ch + 1, chunks), rr, push);
+  nfix[1] = static_cast<int64_t>(items.size());
+  for (uint64_t e : items) {
+    int64_t i, j;
+    bool w;
+    unpack_host_item(e, i, j, w);
+    st[i] |= radial_fixup_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, j, w);
+  }
+}
+
+}  // namespace kg

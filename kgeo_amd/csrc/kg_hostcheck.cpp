@@ -15,6 +15,7 @@
 #include "kg_dual.hpp"
 #include "kg_equatorial.hpp"
 #include "kg_geodesic.hpp"
+#include "kg_host_passes.hpp"
 #include "kg_sample.hpp"
 
 using namespace kg;
@@ -34,6 +35,7 @@ RayScalars<double> scalars_from(const double* s) {
 }
 
 }  // namespace
+
 
 extern "C" {
 
@@ -101,16 +103,22 @@ void kgh_setup(const double* sc, const double* alpha, const double* beta, int64_
   }
 }
 
+
 // Full raytrace of the kernels' algorithm on the CPU: the same plan layout
-// ([PL_N][npix] slot-major), image-wide flags and per-pixel sample loops
-// (kg_sample.hpp: theta_pixel then radial_pixel) as kg_setup / kg_theta /
-// kg_radial, with ngeo split into `chunks` contiguous ranges per pixel as the
-// kernels' grid does.  Outputs [ngeo][npix]; tau_in (optional, [ngeo][npix])
-// replaces the linspace grid, which also makes every sample a direct
-// evaluation.  status (optional) receives the kernels' per-pixel bits.
-void kgh_raytrace_loops(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo,
-                        int do_phi_t, const double* tau_in, int chunks, double* tau, double* t, double* r, double* th,
-                        double* ph, double* sig, int32_t* nmax_eq, uint32_t* status) {
+// ([PL_N][npix] slot-major), image-wide flags and per-pixel sample code
+// (kg_sample.hpp) as kg_setup and the sample kernels, with ngeo split into
+// `chunks` contiguous ranges per pixel as the kernels' grid does.
+//   passes = 3: the anchor / walk / fixup passes (the kernels' path for the
+//               linspace grid with ngeo > 2); nfix[2] (optional) receives the
+//               theta and radial fixup-queue lengths;
+//   passes = 1: one combined loop per coordinate (theta_pixel, radial_pixel:
+//               the kernels' path for caller-given Mino times).
+// Outputs [ngeo][npix]; tau_in (optional, [ngeo][npix]) replaces the linspace
+// grid, which also makes every sample a direct evaluation.  status (optional)
+// receives the kernels' per-pixel bits.
+void kgh_raytrace_passes(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo,
+                         int do_phi_t, const double* tau_in, int chunks, int passes, double* tau, double* t, double* r,
+                         double* th, double* ph, double* sig, int32_t* nmax_eq, uint32_t* status, int64_t* nfix) {
   RayScalars<double> c = scalars_from(sc);
   std::vector<double> plan(static_cast<size_t>(npix) * PL_N);
   std::vector<int8_t> rcl(npix), tcl(npix);
@@ -131,6 +139,18 @@ void kgh_raytrace_loops(const double* sc, const double* alpha, const double* bet
     o.t = nullptr;
     o.ph = nullptr;
   }
+  if (passes == 3 && !tau_in && G > 2) {
+    int64_t nf[2] = {0, 0};
+    if (do_phi_t) three_passes<true, double>(c, plan.data(), gflags, o, chunks, st.data(), nf);
+    else three_passes<false, double>(c, plan.data(), gflags, o, chunks, st.data(), nf);
+    if (nfix) {
+      nfix[0] = nf[0];
+      nfix[1] = nf[1];
+    }
+    if (status)
+      for (int64_t i = 0; i < npix; ++i) status[i] = st[i];
+    return;
+  }
   for (int64_t i = 0; i < npix; ++i) {
     for (int ch = 0; ch < chunks; ++ch) {
       const int64_t j0 = chunk_start(G, ch, chunks), j1 = chunk_start(G, ch + 1, chunks);
@@ -147,6 +167,13 @@ void kgh_raytrace_loops(const double* sc, const double* alpha, const double* bet
   }
   if (status)
     for (int64_t i = 0; i < npix; ++i) status[i] = st[i];
+}
+
+void kgh_raytrace_loops(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo,
+                        int do_phi_t, const double* tau_in, int chunks, double* tau, double* t, double* r, double* th,
+                        double* ph, double* sig, int32_t* nmax_eq, uint32_t* status) {
+  kgh_raytrace_passes(sc, alpha, beta, npix, ngeo, do_phi_t, tau_in, chunks, 3, tau, t, r, th, ph, sig, nmax_eq,
+                      status, nullptr);
 }
 
 // Backwards-compatible entry: all samples of a pixel in one range.

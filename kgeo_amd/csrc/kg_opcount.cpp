@@ -59,6 +59,7 @@ inline CD kpow(const CD& x, double y) { ++g_ops[POW]; return CD(std::pow(x.v, y)
 }  // namespace kgcount
 
 #include "kg_geodesic.hpp"
+#include "kg_host_passes.hpp"
 #include "kg_sample.hpp"
 
 using kgcount::CD;
@@ -70,11 +71,11 @@ extern "C" {
 int kgc_num_ops(void) { return kgcount::NOPS; }
 
 // Per-sample op counts summed over npix pixels x ngeo linspace samples, as
-// the sample kernels execute them: the same per-pixel loops (kg_sample.hpp:
-// walking with direct evaluations at chunk starts, anchors, the last sample
-// and refused steps) with ngeo split into `chunks` ranges per pixel as the
-// kernels' grid does.  Separately, the per-pixel setup counts.  sc as in
-// kg_hostcheck.cpp.
+// the sample kernels execute them: the anchor / walk / fixup passes
+// (kg_host_passes.hpp: direct evaluations at the step, the segment starts,
+// the last sample and from each refused step on; the walk elsewhere) with
+// ngeo split into `chunks` ranges per pixel as the kernels' grid does.
+// Separately, the per-pixel setup counts.  sc as in kg_hostcheck.cpp.
 void kgc_count(const double* sc, const double* alpha, const double* beta, int64_t npix, int64_t ngeo, int chunks,
                uint64_t* per_sample_ops, uint64_t* setup_ops) {
   RayScalars<double> c = make_scalars(sc[0], sc[2], sc[1], sc[3]);
@@ -108,14 +109,9 @@ void kgc_count(const double* sc, const double* alpha, const double* beta, int64_
   SampleOut o{nullptr, nullptr, t.data(), r.data(), th.data(), ph.data(), nullptr, npix, ngeo};
   if (chunks < 1) chunks = 1;
   std::memset(kgcount::g_ops, 0, sizeof(kgcount::g_ops));
-  for (int64_t i = 0; i < npix; ++i) {
-    for (int ch = 0; ch < chunks; ++ch) {
-      const int64_t j0 = chunk_start(ngeo, ch, chunks), j1 = chunk_start(ngeo, ch + 1, chunks);
-      if (j0 >= j1) continue;
-      theta_pixel<true, CD>(plan.data(), npix, i, gflags, o, i, j0, j1);
-      radial_pixel<true, CD>(cc, plan.data(), npix, i, gflags, o, i, j0, j1);
-    }
-  }
+  std::vector<uint32_t> st(static_cast<size_t>(npix), 0u);
+  int64_t nfix[2];
+  three_passes<true, CD>(cc, plan.data(), gflags, o, chunks, st.data(), nfix);
   for (int q = 0; q < kgcount::NOPS; ++q) samp_acc[q] += kgcount::g_ops[q];
   for (int q = 0; q < kgcount::NOPS; ++q) {
     per_sample_ops[q] = samp_acc[q];

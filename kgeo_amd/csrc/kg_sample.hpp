@@ -336,6 +336,428 @@ KG_INLINE uint32_t radial_pixel(const RayScalars<T>& c, const double* __restrict
   return loop_status(bad, lad);
 }
 
+// ---------------------------------------------------------------------------
+// The walk in three passes.  Each is its own kernel, so the walking loop is
+// not held to the register budget of the direct evaluation (Landen ladder,
+// Carlson duplication for three characteristics), which it needs only at a
+// segment's first sample:
+//   anchor  one lane per (pixel, record): the direct evaluation at one
+//           segment's first sample, at the last grid sample, or (record 0) at
+//           the step h itself; stores the walk state in the record table and,
+//           for a sample, its outputs;
+//   walk    one lane per (pixel, sample chunk): every other sample from the
+//           addition theorems alone, restarting from the records; a step that
+//           is not well conditioned queues its segment for the fixup pass and
+//           the lane resumes at the next segment;
+//   fixup   one lane per queued (pixel, segment): the combined loops above
+//           (direct evaluation wherever the walk cannot step) over the whole
+//           segment.  Pixels the walk does not cover (theta classes other
+//           than ordinary and vortical, radial m >= 1, numpy's zero-step
+//           grid, ngeo <= 2) queue all their segments.
+// A segment is kAnchorEveryLong samples (theta, radial case 2) or kAnchorEvery
+// samples (radial cases 3 and 4) starting at a multiple of its length, so the
+// direct evaluations fall on the samples the combined loops pick, whatever
+// the chunking.  Records are slot-major [record][field][stride]: record 0 the
+// step, record 1 + k the sample k * (theta: kAnchorEveryLong, radial:
+// kAnchorEvery), the last record the sample ngeo - 1.
+constexpr int kThFields = 6;   // s, c, d, D, P, Lg
+constexpr int kRFields = 13;   // s, c, d, D, P[3], Lg[3], Lb[3] (walked R1/S1 logs)
+
+KG_INLINE int64_t seg_count(int64_t ngeo, int64_t len) { return (ngeo + len - 1) / len; }
+KG_INLINE int64_t theta_records(int64_t ngeo) { return seg_count(ngeo, kAnchorEveryLong) + 2; }
+KG_INLINE int64_t radial_records(int64_t ngeo) { return seg_count(ngeo, kAnchorEvery) + 2; }
+
+struct Records {
+  double* p;
+  int64_t stride;
+  KG_INLINE double* at(int64_t rec, int nf, int f, int64_t col) const { return p + (rec * nf + f) * stride + col; }
+};
+
+template <class T, int NP>
+KG_INLINE void record_store(const Records& rt, int64_t rec, int nf, int64_t col, const WalkState<T, NP>& st,
+                            const T* lb) {
+  *rt.at(rec, nf, 0, col) = val(st.s);
+  *rt.at(rec, nf, 1, col) = val(st.c);
+  *rt.at(rec, nf, 2, col) = val(st.d);
+  *rt.at(rec, nf, 3, col) = val(st.D);
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    *rt.at(rec, nf, 4 + q, col) = val(st.P[q]);
+    *rt.at(rec, nf, 4 + NP + q, col) = val(st.Lg[q]);
+    if (lb) *rt.at(rec, nf, 4 + 2 * NP + q, col) = val(lb[q]);
+  }
+}
+
+template <class T, int NP>
+KG_INLINE void record_load(const Records& rt, int64_t rec, int nf, int64_t col, WalkState<T, NP>& st, T* lb) {
+  st.s = T(*rt.at(rec, nf, 0, col));
+  st.c = T(*rt.at(rec, nf, 1, col));
+  st.d = T(*rt.at(rec, nf, 2, col));
+  st.D = T(*rt.at(rec, nf, 3, col));
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    st.P[q] = T(*rt.at(rec, nf, 4 + q, col));
+    st.Lg[q] = T(*rt.at(rec, nf, 4 + NP + q, col));
+    if (lb) lb[q] = T(*rt.at(rec, nf, 4 + 2 * NP + q, col));
+  }
+}
+
+// Sample of record rec >= 1 whose segments are `len` samples.
+KG_INLINE int64_t record_sample(int64_t rec, int64_t ngeo, int64_t len) {
+  return (rec == seg_count(ngeo, len) + 1) ? ngeo - 1 : (rec - 1) * len;
+}
+
+// Theta record rec of output pixel i (direct evaluation).
+template <class T, int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_anchor(const SampleOut& o, const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, int64_t i,
+                            int64_t rec, double stop, const Records& rt, int64_t col) {
+  const T m = tc.v[TH_k];
+  WalkStep<T, 1> w;
+  const T n[1] = {tc.v[TH_n]};
+  walk_init(w, m, n);
+  const bool step = (rec == 0);
+  const int64_t j = step ? 0 : record_sample(rec, o.ngeo, kAnchorEveryLong);
+  const double tau = step ? 0.0 : mino_tau(j, o.ngeo, stop, false);
+  const T arg = step ? -tc.v[TH_w] * (stop / static_cast<double>(o.ngeo - 1)) : theta_arg(tc, T(tau));
+  ThState<T> ts;
+  theta_state<T, KIND>(tc, lad, arg, DO_PHI_T, ts);
+  WalkState<T, 1> st;
+  st.s = ts.s;
+  st.c = ts.c;
+  st.d = ts.d;
+  st.D = DO_PHI_T ? ts.EmF / m : T(0.0);
+  st.P[0] = T(0.0);
+  st.Lg[0] = T(0.0);
+  if (DO_PHI_T) walk_enter(w, 0, ts.Pi, arg, ts.s, ts.c, ts.d, st);
+  record_store<T, 1>(rt, rec, kThFields, col, st, static_cast<const T*>(nullptr));
+  if (step) return false;
+  T th = T(0.0), Gph = T(0.0), Gt = T(0.0);
+  theta_out<T, KIND>(tc, arg, ts, DO_PHI_T, th, Gph, Gt);
+  return theta_store<T, DO_PHI_T>(o, j * o.npix + i, tau, th, Gph, Gt);
+}
+
+// Theta samples j0..j1-1 walked from the records.  Where a step is refused,
+// `push(i, j, true)` queues samples j.. of the segment for the fixup pass,
+// which evaluates j directly and walks on from there, as the combined loop
+// does; the lane idles until the next segment (j stays uniform).
+template <class T, int KIND, bool DO_PHI_T, class Push>
+KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t i, int64_t j0, int64_t j1,
+                          double stop, const Records& rt, int64_t col, Push& push) {
+  constexpr int64_t L = kAnchorEveryLong;
+  const T m = tc.v[TH_k];
+  WalkStep<T, 1> w;
+  const T n[1] = {tc.v[TH_n]};
+  walk_init(w, m, n);
+  record_load<T, 1>(rt, 0, kThFields, col, w.b, static_cast<T*>(nullptr));
+  const int64_t last = o.ngeo - 1;
+  WalkState<T, 1> st;
+  bool bad = false;
+  bool queued = false;  // the rest of this segment is queued
+  for (int64_t j = j0; j < j1; ++j) {
+    if (j == last) continue;  // written by the anchor pass
+    if (j % L == 0) {
+      record_load<T, 1>(rt, 1 + j / L, kThFields, col, st, static_cast<T*>(nullptr));
+      queued = false;
+      continue;
+    }
+    if (queued) continue;
+    WalkState<T, 1> nx;
+    if (!walk_step<T, 1, DO_PHI_T>(st, w, nx)) {
+      push(i, j, true);
+      queued = true;
+      continue;
+    }
+    st = nx;
+    const int64_t idx = j * o.npix + i;
+    const double tau = mino_tau(j, o.ngeo, stop, false);
+    const T snarg = theta_arg(tc, T(tau));
+    ThState<T> ts;
+    ts.s = st.s;
+    ts.c = st.c;
+    ts.d = st.d;
+    ts.F = snarg;
+    ts.EmF = m * st.D;
+    ts.Pi = DO_PHI_T ? walk_pi(w, 0, st, snarg) : T(0.0);
+    T th = T(0.0), Gph = T(0.0), Gt = T(0.0);
+    theta_out<T, KIND>(tc, snarg, ts, DO_PHI_T, th, Gph, Gt);
+    bad |= theta_store<T, DO_PHI_T>(o, idx, tau, th, Gph, Gt);
+  }
+  return bad;
+}
+
+// Radial record rec of output pixel i (direct evaluation); case 2 keeps only
+// the records on its longer segments.
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_anchor(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
+                             const CombineConsts<T>& cmb, int64_t i, int64_t rec, double stop, const Records& rt,
+                             int64_t col) {
+  constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
+  constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
+  const bool step = (rec == 0);
+  const int64_t j = step ? 0 : record_sample(rec, o.ngeo, kAnchorEvery);
+  if (!step && j != o.ngeo - 1 && j % L != 0) return false;
+  const T m = rc.v[R_k];
+  WalkStep<T, 3> w;
+  {
+    T n[3], pc[3];
+    radial_pi_params<T, CASE>(rc, n, pc);
+    walk_init(w, m, n);
+  }
+  const double tau = step ? 0.0 : mino_tau(j, o.ngeo, stop, false);
+  T X, dv;
+  if (step) {
+    X = -rc.v[R_Xs] * (stop / static_cast<double>(o.ngeo - 1));
+    dv = T(INFINITY);  // plain Jacobi functions at the step
+  } else {
+    dv = rc.v[R_Xs] * tau;
+    X = rc.v[R_U0] - dv;
+  }
+  RState<T> rs;
+  radial_state<T, CASE>(rc, lad, X, dv, DO_PHI_T, rs);
+  WalkState<T, 3> st;
+  T lb[3];
+  st.s = rs.s;
+  st.c = rs.c;
+  st.d = rs.d;
+  st.D = DO_PHI_T ? rs.EmF / m : T(0.0);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    st.P[q] = T(0.0);
+    st.Lg[q] = T(0.0);
+    if (DO_PHI_T) walk_enter(w, q, rs.Pi[q], X, rs.s, rs.c, rs.d, st);
+    lb[q] = (kBlocks && !step) ? rs.lg[q] : T(0.0);
+  }
+  record_store<T, 3>(rt, rec, kRFields, col, st, lb);
+  if (step) return false;
+  RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
+  return radial_store<T, DO_PHI_T>(o, cmb, j * o.npix + i, tau, ro);
+}
+
+// Radial samples j0..j1-1 walked from the records.
+template <class T, int CASE, bool DO_PHI_T, class Push>
+KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const CombineConsts<T>& cmb, int64_t i,
+                           int64_t j0, int64_t j1, double stop, const Records& rt, int64_t col, Push& push) {
+  constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
+  constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
+  const T m = rc.v[R_k];
+  WalkStep<T, 3> w;
+  {
+    T n[3], pc[3];
+    radial_pi_params<T, CASE>(rc, n, pc);
+    walk_init(w, m, n);
+  }
+  record_load<T, 3>(rt, 0, kRFields, col, w.b, static_cast<T*>(nullptr));
+  const int64_t last = o.ngeo - 1;
+  WalkState<T, 3> st;
+  T Lb[3];
+  bool bad = false;
+  bool queued = false;
+  for (int64_t j = j0; j < j1; ++j) {
+    if (j == last) continue;  // written by the anchor pass
+    if (j % L == 0) {
+      record_load<T, 3>(rt, 1 + j / kAnchorEvery, kRFields, col, st, kBlocks ? Lb : static_cast<T*>(nullptr));
+      queued = false;
+      continue;
+    }
+    if (queued) continue;
+    WalkState<T, 3> nx;
+    if (!walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
+      push(i, j, true);
+      queued = true;
+      continue;
+    }
+    RState<T> rs;
+    if (kBlocks) {
+      T a0[3], b0[3], a1[3], b1[3], den[3], iden[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        block_log_args<T, CASE>(rc, q, st.s, st.d, a0[q], b0[q]);
+        block_log_args<T, CASE>(rc, q, nx.s, nx.d, a1[q], b1[q]);
+        den[q] = walk_log_den(a0[q], b0[q], a1[q], b1[q]);
+      }
+      shared_reciprocals<T, 3>(den, iden);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        Lb[q] = walk_log(Lb[q], a0[q], b0[q], a1[q], b1[q], iden[q]);
+        rs.lg[q] = Lb[q];
+      }
+    }
+    st = nx;
+    const int64_t idx = j * o.npix + i;
+    const double tau = mino_tau(j, o.ngeo, stop, false);
+    const T X = rc.v[R_U0] - rc.v[R_Xs] * tau;
+    rs.s = st.s;
+    rs.c = st.c;
+    rs.d = st.d;
+    rs.F = X;
+    rs.EmF = m * st.D;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) rs.Pi[q] = DO_PHI_T ? walk_pi(w, q, st, X) : T(0.0);
+    RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
+    bad |= radial_store<T, DO_PHI_T>(o, cmb, idx, tau, ro);
+  }
+  return bad;
+}
+
+// Per-pixel dispatch of the three passes.  A theta pixel walks when its
+// motion is ordinary or vortical; a radial pixel when its modulus is below 1.
+struct ThetaPick {
+  int kind;
+  double stop;
+  bool walk;
+};
+
+template <class T>
+KG_INLINE ThetaPick theta_pick(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                               const SampleOut& o, ThetaConsts<T>& tc) {
+#pragma unroll
+  for (int s = 0; s < TH_NSLOT; ++s) tc.v[s] = T(plan[static_cast<int64_t>(PL_TH + s) * stride + p]);
+  const int kind = static_cast<int>(plan[static_cast<int64_t>(PL_THK) * stride + p]);
+  const bool eta0 = plan[static_cast<int64_t>(PL_ETA0) * stride + p] != 0.0;
+  ThetaPick tp;
+  tp.stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
+  tp.kind = (eta0 && (gflags & GF_ANY_ETA_POS) == 0) ? THK_NONE : kind;
+  tp.walk = walk_ok(o, (gflags & GF_ANY_STEP_ZERO) != 0) && (tp.kind == THK_ORD || tp.kind == THK_VORT);
+  return tp;
+}
+
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t theta_anchor_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                      const SampleOut& o, int64_t i, int64_t rec, const Records& rt) {
+  ThetaConsts<T> tc;
+  const ThetaPick tp = theta_pick(plan, stride, p, gflags, o, tc);
+  if (!tp.walk) return 0u;
+  JacobiLadder<T> lad;
+  jacobi_ladder(tc.v[TH_mp], lad);
+  const bool bad = (tp.kind == THK_ORD) ? theta_anchor<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, rec, tp.stop, rt, p)
+                                        : theta_anchor<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, rec, tp.stop, rt, p);
+  return loop_status(bad, lad);
+}
+
+template <bool DO_PHI_T, class Push, class T = double>
+KG_INLINE uint32_t theta_walk_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                    const SampleOut& o, int64_t i, int64_t j0, int64_t j1, const Records& rt,
+                                    Push& push) {
+  ThetaConsts<T> tc;
+  const ThetaPick tp = theta_pick(plan, stride, p, gflags, o, tc);
+  if (!tp.walk) {
+    for (int64_t js = j0; js < j1; js += kAnchorEveryLong) push(i, js, false);
+    return 0u;
+  }
+  const bool bad = (tp.kind == THK_ORD)
+                       ? theta_walk<T, THK_ORD, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push)
+                       : theta_walk<T, THK_VORT, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push);
+  return bad ? ST_NONFINITE : 0u;
+}
+
+// End of the fixup range that starts at sample j inside a segment of `len`
+// samples: the segment's end, short of the last grid sample when the anchor
+// pass wrote it (walked: the radial store combines in place, so no sample may
+// be stored twice).
+KG_INLINE int64_t fixup_end(int64_t j, int64_t len, int64_t ngeo, bool walked) {
+  const int64_t je = (j / len + 1) * len;
+  const int64_t lim = walked ? ngeo - 1 : ngeo;
+  return je < lim ? je : lim;
+}
+
+// Queued theta samples j.. to the end of their segment.
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t theta_fixup_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                     const SampleOut& o, int64_t i, int64_t j, bool walked) {
+  return theta_pixel<DO_PHI_T, T>(plan, stride, p, gflags, o, i, j, fixup_end(j, kAnchorEveryLong, o.ngeo, walked));
+}
+
+template <class T>
+KG_INLINE int radial_pick(const double* __restrict__ plan, int64_t stride, int64_t p, RadialConsts<T>& rc, T& lam,
+                          double& stop) {
+#pragma unroll
+  for (int s = 0; s < R_NSLOT; ++s) rc.v[s] = T(plan[static_cast<int64_t>(PL_R + s) * stride + p]);
+  rc.rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
+  lam = T(plan[static_cast<int64_t>(PL_LAM) * stride + p]);
+  stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
+  return (rc.rcase == 3 || rc.rcase == 4) ? kAnchorEvery : kAnchorEveryLong;
+}
+
+KG_INLINE bool radial_walks(const SampleOut& o, uint32_t gflags, double m) {
+  return walk_ok(o, (gflags & GF_ANY_STEP_ZERO) != 0) && m < 1.0;
+}
+
+// The anchor of one radial case (slots loaded inside the case, as in the walk).
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE uint32_t radial_anchor_case(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                      int64_t p, const SampleOut& o, int64_t i, int64_t rec, const Records& rt) {
+  RadialConsts<T> rc;
+  T lam;
+  double stop;
+  radial_pick(plan, stride, p, rc, lam, stop);
+  const CombineConsts<T> cmb = combine_consts(c, lam);
+  JacobiLadder<T> lad;
+  jacobi_ladder(rc.v[R_k], lad);
+  const bool bad = radial_anchor<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, rec, stop, rt, p);
+  return loop_status(bad, lad);
+}
+
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_anchor_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t rec,
+                                       const Records& rt) {
+  const int rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
+  const double m = plan[static_cast<int64_t>(PL_R + R_k) * stride + p];
+  if (!radial_walks(o, gflags, m)) return 0u;
+  // case 2 keeps every other record (its segments are twice as long)
+  const int64_t j = (rec == 0) ? 0 : record_sample(rec, o.ngeo, kAnchorEvery);
+  if (rcase != 3 && rcase != 4 && rec != 0 && j != o.ngeo - 1 && j % kAnchorEveryLong != 0) return 0u;
+  switch (rcase) {
+    case 4: return radial_anchor_case<T, 4, DO_PHI_T>(c, plan, stride, p, o, i, rec, rt);
+    case 3: return radial_anchor_case<T, 3, DO_PHI_T>(c, plan, stride, p, o, i, rec, rt);
+    default: return radial_anchor_case<T, 2, DO_PHI_T>(c, plan, stride, p, o, i, rec, rt);
+  }
+}
+
+// The walk of one radial case: its plan slots are loaded here, inside the
+// case, so the three cases' constants are never live together.
+template <class T, int CASE, bool DO_PHI_T, class Push>
+KG_INLINE bool radial_walk_case(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride, int64_t p,
+                                const SampleOut& o, int64_t i, int64_t j0, int64_t j1, const Records& rt,
+                                Push& push) {
+  RadialConsts<T> rc;
+  T lam;
+  double stop;
+  radial_pick(plan, stride, p, rc, lam, stop);
+  const CombineConsts<T> cmb = combine_consts(c, lam);
+  return radial_walk<T, CASE, DO_PHI_T>(o, rc, cmb, i, j0, j1, stop, rt, p, push);
+}
+
+template <bool DO_PHI_T, class Push, class T = double>
+KG_INLINE uint32_t radial_walk_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                     int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j0,
+                                     int64_t j1, const Records& rt, Push& push) {
+  const int rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
+  const double m = plan[static_cast<int64_t>(PL_R + R_k) * stride + p];
+  if (!radial_walks(o, gflags, m)) {
+    const int len = (rcase == 3 || rcase == 4) ? kAnchorEvery : kAnchorEveryLong;
+    for (int64_t js = j0; js < j1; js += len) push(i, js, false);
+    return 0u;
+  }
+  bool bad;
+  switch (rcase) {
+    case 4: bad = radial_walk_case<T, 4, DO_PHI_T>(c, plan, stride, p, o, i, j0, j1, rt, push); break;
+    case 3: bad = radial_walk_case<T, 3, DO_PHI_T>(c, plan, stride, p, o, i, j0, j1, rt, push); break;
+    default: bad = radial_walk_case<T, 2, DO_PHI_T>(c, plan, stride, p, o, i, j0, j1, rt, push); break;
+  }
+  return bad ? ST_NONFINITE : 0u;
+}
+
+// Queued radial samples j.. to the end of their segment.
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_fixup_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                      int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j,
+                                      bool walked) {
+  const int64_t rcase = static_cast<int64_t>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
+  const int64_t len = (rcase == 3 || rcase == 4) ? kAnchorEvery : kAnchorEveryLong;
+  return radial_pixel<DO_PHI_T, T>(c, plan, stride, p, gflags, o, i, j, fixup_end(j, len, o.ngeo, walked));
+}
+
 // Per-pixel part of the plan kernels: the plan column, labels and the two
 // image-wide predicates the reference's numpy code evaluates over all pixels.
 struct SetupOut {

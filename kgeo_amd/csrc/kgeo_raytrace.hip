@@ -71,6 +71,18 @@ __global__ void kg_clear_flags(uint32_t* flags) {
   if (threadIdx.x < kFlagBytes / sizeof(uint32_t)) flags[threadIdx.x] = 0u;
 }
 
+// Word 0 of the flag page holds the image-wide flags (kg_setup); words 1 and
+// 2 count the theta and radial fixup items of one sample pass.
+constexpr int kThetaCount = 1;
+constexpr int kRadialCount = 2;
+
+__global__ void kg_clear_counters(uint32_t* flags) {
+  if (threadIdx.x == 0) {
+    flags[kThetaCount] = 0u;
+    flags[kRadialCount] = 0u;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void kg_setup(DevParams p, const double* __restrict__ alpha,
                                                    const double* __restrict__ beta, double* __restrict__ plan,
                                                    uint32_t* __restrict__ gflags, int8_t* __restrict__ rclass,
@@ -116,6 +128,104 @@ __global__ __launch_bounds__(kBlock) void kg_radial(DevParams p, SampleArgs a) {
   const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
   const uint32_t st = radial_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
   if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+// ---------------------------------------------------------------------------
+// The walk in three passes per coordinate (kg_sample.hpp): anchor -> walk ->
+// fixup for theta, then the same for r (whose stores combine phi, t, sigma
+// with the theta parts).  The fixup queue holds one 64-bit item per queued
+// range: walked << 63 | first sample << 32 | pixel.
+// ---------------------------------------------------------------------------
+struct FixupQueue {
+  unsigned long long* items;
+  uint32_t* count;
+  unsigned long long cap;
+};
+
+struct DevPush {
+  FixupQueue q;
+  __device__ void operator()(int64_t i, int64_t j, bool walked) const {
+    const uint32_t s = atomicAdd(q.count, 1u);
+    if (s < q.cap)
+      q.items[s] = (walked ? (1ull << 63) : 0ull) | (static_cast<unsigned long long>(j) << 32) |
+                   static_cast<unsigned long long>(static_cast<uint32_t>(i));
+  }
+};
+
+__device__ inline void unpack_item(unsigned long long e, int64_t& i, int64_t& j, bool& walked) {
+  i = static_cast<int64_t>(e & 0xffffffffull);
+  j = static_cast<int64_t>((e >> 32) & 0x7fffffffull);
+  walked = (e >> 63) != 0ull;
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_theta_anchor(DevParams p, SampleArgs a, Records rt, int64_t nrec) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  uint32_t st = 0u;
+  for (int64_t rec = blockIdx.y; rec < nrec; rec += gridDim.y)
+    st |= theta_anchor_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, rec, rt);
+  if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_theta_walk(DevParams p, SampleArgs a, Records rt, FixupQueue q) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
+  const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
+  DevPush push{q};
+  const uint32_t st = theta_walk_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1, rt, push);
+  if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_theta_fixup(DevParams p, SampleArgs a, FixupQueue q) {
+  const unsigned long long c = *q.count;
+  const unsigned long long n = c < q.cap ? c : q.cap;
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+    int64_t i, j;
+    bool walked;
+    unpack_item(q.items[e], i, j, walked);
+    const uint32_t st = theta_fixup_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j, walked);
+    if (a.status && st) atomicOr(&a.status[i], st);
+  }
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_radial_anchor(DevParams p, SampleArgs a, Records rt, int64_t nrec) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  uint32_t st = 0u;
+  for (int64_t rec = blockIdx.y; rec < nrec; rec += gridDim.y)
+    st |= radial_anchor_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, rec, rt);
+  if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void kg_radial_walk(DevParams p, SampleArgs a, Records rt, FixupQueue q) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
+  const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
+  DevPush push{q};
+  const uint32_t st = radial_walk_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1, rt, push);
+  if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_radial_fixup(DevParams p, SampleArgs a, FixupQueue q) {
+  const unsigned long long c = *q.count;
+  const unsigned long long n = c < q.cap ? c : q.cap;
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+    int64_t i, j;
+    bool walked;
+    unpack_item(q.items[e], i, j, walked);
+    const uint32_t st = radial_fixup_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j, walked);
+    if (a.status && st) atomicOr(&a.status[i], st);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void kg_ellip_pi(const double* __restrict__ n, const double* __restrict__ phi,
@@ -198,14 +308,44 @@ int env_chunks() {
   return s ? std::atoi(s) : 0;
 }
 
-int validate(const kgeo_ray_params* p, const double* alpha, const double* beta, void* ws, size_t wsb) {
+// Workspace: the flag page, the plan [PL_N][npix] and, for a walked grid
+// (ngeo > 2), the theta and radial record tables and the fixup queue (one
+// item per 16-sample segment and pixel bounds every pass's pushes).
+struct WsLayout {
+  size_t plan, threc, rrec, queue, total;
+  int64_t nth, nr;
+  unsigned long long qcap;
+};
+
+WsLayout ws_layout(int64_t npix, int64_t ngeo) {
+  WsLayout L{};
+  const size_t np = static_cast<size_t>(npix > 0 ? npix : 0);
+  L.plan = kFlagBytes;
+  size_t off = kFlagBytes + np * PL_N * sizeof(double);
+  if (ngeo > 2) {
+    L.nth = theta_records(ngeo);
+    L.nr = radial_records(ngeo);
+    L.threc = off;
+    off += static_cast<size_t>(L.nth) * kThFields * np * sizeof(double);
+    L.rrec = off;
+    off += static_cast<size_t>(L.nr) * kRFields * np * sizeof(double);
+    L.qcap = static_cast<unsigned long long>(np) * static_cast<unsigned long long>(seg_count(ngeo, kAnchorEvery));
+    L.queue = off;
+    off += static_cast<size_t>(L.qcap) * sizeof(unsigned long long);
+  }
+  L.total = off;
+  return L;
+}
+
+int validate(const kgeo_ray_params* p, const double* alpha, const double* beta, void* ws, size_t wsb,
+             int64_t ngeo) {
   if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
   if (p->npix < 0) return fail(KGEO_E_ARG, "npix < 0%s (%lld)", "", p->npix);
   if (p->npix > 0 && (!alpha || !beta)) return fail(KGEO_E_ARG, "alpha/beta is NULL%s%lld");
   if (p->npix > 0 && !ws) return fail(KGEO_E_WORKSPACE, "workspace is NULL%s%lld");
-  if (wsb < kgeo_workspace_bytes(p->npix))
-    return fail(KGEO_E_WORKSPACE, "workspace too small%s: need %lld bytes", "",
-                static_cast<long long>(kgeo_workspace_bytes(p->npix)));
+  const size_t need = kgeo_workspace_bytes(p->npix, ngeo);
+  if (wsb < need)
+    return fail(KGEO_E_WORKSPACE, "workspace too small%s: need %lld bytes", "", static_cast<long long>(need));
   if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) return fail(KGEO_E_WORKSPACE, "workspace not 256-byte aligned%s%lld");
   return KGEO_OK;
 }
@@ -225,8 +365,39 @@ int run_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, 
   return check_launch("kg_setup");
 }
 
+template <bool DO_PHI_T>
+int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipStream_t st) {
+  const WsLayout L = ws_layout(d.npix, d.ngeo);
+  char* base = static_cast<char*>(ws);
+  uint32_t* flags = static_cast<uint32_t*>(ws);
+  const Records rth{reinterpret_cast<double*>(base + L.threc), d.npix};
+  const Records rr{reinterpret_cast<double*>(base + L.rrec), d.npix};
+  unsigned long long* items = reinterpret_cast<unsigned long long*>(base + L.queue);
+  const FixupQueue qth{items, flags + kThetaCount, L.qcap};
+  const FixupQueue qr{items, flags + kRadialCount, L.qcap};
+  const unsigned nb = grid.x;
+  const unsigned long long fb = (L.qcap + kBlock - 1) / kBlock;
+  const dim3 fgrid(static_cast<unsigned>(fb < 1024ull ? fb : 1024ull));
+  auto recs = [](int64_t n) { return static_cast<unsigned>(n < 65535 ? n : 65535); };
+  hipLaunchKernelGGL(kg_clear_counters, dim3(1), dim3(64), 0, st, flags);
+  int rc = check_launch("kg_clear_counters");
+  if (rc) return rc;
+  hipLaunchKernelGGL(kg_theta_anchor<DO_PHI_T>, dim3(nb, recs(L.nth)), dim3(kBlock), 0, st, d, a, rth, L.nth);
+  if ((rc = check_launch("kg_theta_anchor"))) return rc;
+  hipLaunchKernelGGL(kg_theta_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rth, qth);
+  if ((rc = check_launch("kg_theta_walk"))) return rc;
+  hipLaunchKernelGGL(kg_theta_fixup<DO_PHI_T>, fgrid, dim3(kBlock), 0, st, d, a, qth);
+  if ((rc = check_launch("kg_theta_fixup"))) return rc;
+  hipLaunchKernelGGL(kg_radial_anchor<DO_PHI_T>, dim3(nb, recs(L.nr)), dim3(kBlock), 0, st, d, a, rr, L.nr);
+  if ((rc = check_launch("kg_radial_anchor"))) return rc;
+  hipLaunchKernelGGL(kg_radial_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rr, qr);
+  if ((rc = check_launch("kg_radial_walk"))) return rc;
+  hipLaunchKernelGGL(kg_radial_fixup<DO_PHI_T>, fgrid, dim3(kBlock), 0, st, d, a, qr);
+  return check_launch("kg_radial_fixup");
+}
+
 int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, double* tau, double* t, double* r,
-               double* th, double* ph, double* sig, uint32_t* status, const void* ws, hipStream_t st) {
+               double* th, double* ph, double* sig, uint32_t* status, void* ws, hipStream_t st) {
   DevParams d = make_dev(p, ngeo);
   if (d.npix == 0 || ngeo <= 0) return KGEO_OK;
   if (d.do_phi_t && (!t || !ph))
@@ -237,6 +408,10 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   const int chunks = kgeo_sample_chunks(d.npix, ngeo);
   dim3 grid(nb, static_cast<unsigned>(chunks));
   SampleArgs a{plan, gflags, SampleOut{tau_in, tau, t, r, th, ph, sig, d.npix, ngeo}, status, chunks};
+  // the linspace grid walks (three passes); caller-given Mino times and
+  // grids of at most two samples are evaluated directly in one pass
+  if (!tau_in && ngeo > 2 && std::getenv("KGEO_ONE_PASS") == nullptr)
+    return d.do_phi_t ? run_walk<true>(d, a, ws, grid, st) : run_walk<false>(d, a, ws, grid, st);
   if (d.do_phi_t) {
     hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);
     int rc = check_launch("kg_theta");
@@ -278,27 +453,29 @@ int kgeo_sample_chunks(int64_t npix, int64_t ngeo) {
   return c;
 }
 
-size_t kgeo_workspace_bytes(int64_t npix) {
+size_t kgeo_workspace_bytes(int64_t npix, int64_t ngeo) {
   if (npix < 0) return 0;
-  return kFlagBytes + static_cast<size_t>(npix) * PL_N * sizeof(double);
+  return ws_layout(npix, ngeo).total;
 }
 
 int kgeo_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, int8_t* rclass, int8_t* thclass,
               int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* workspace, size_t workspace_bytes,
               void* stream) {
-  int rc = validate(p, alpha, beta, workspace, workspace_bytes);
+  int rc = validate(p, alpha, beta, workspace, workspace_bytes, 0);
   if (rc) return rc;
   return run_plan(p, alpha, beta, rclass, thclass, nmax_eq, tau_tot, status, workspace,
                   static_cast<hipStream_t>(stream));
 }
 
 int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, double* tau, double* t, double* r,
-                double* th, double* ph, double* sig, uint32_t* status, const void* workspace, size_t workspace_bytes,
+                double* th, double* ph, double* sig, uint32_t* status, void* workspace, size_t workspace_bytes,
                 void* stream) {
   if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
   if (p->npix < 0 || ntau < 0) return fail(KGEO_E_ARG, "negative size%s%lld");
   if (p->npix > 0 && !workspace) return fail(KGEO_E_WORKSPACE, "workspace is NULL%s%lld");
-  if (workspace_bytes < kgeo_workspace_bytes(p->npix)) return fail(KGEO_E_WORKSPACE, "workspace too small%s%lld");
+  const size_t need = kgeo_workspace_bytes(p->npix, tau_in ? 0 : ntau);
+  if (workspace_bytes < need)
+    return fail(KGEO_E_WORKSPACE, "workspace too small%s: need %lld bytes", "", static_cast<long long>(need));
   return run_sample(p, ntau, tau_in, tau, t, r, th, ph, sig, status, workspace, static_cast<hipStream_t>(stream));
 }
 
@@ -306,7 +483,7 @@ int kgeo_raytrace_ana(const kgeo_ray_params* p, const double* alpha, const doubl
                       double* r, double* th, double* ph, double* sig, int8_t* rclass, int8_t* thclass,
                       int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* workspace, size_t workspace_bytes,
                       void* stream) {
-  int rc = validate(p, alpha, beta, workspace, workspace_bytes);
+  int rc = validate(p, alpha, beta, workspace, workspace_bytes, p ? p->ngeo : 0);
   if (rc) return rc;
   if (p->ngeo < 1) return fail(KGEO_E_ARG, "ngeo < 1%s (%lld)", "", p->ngeo);
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -318,7 +495,7 @@ int kgeo_raytrace_ana(const kgeo_ray_params* p, const double* alpha, const doubl
 int kgeo_coords_at_tau(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau_in,
                        int64_t ntau, double* t, double* r, double* th, double* ph, double* sig, uint32_t* status,
                        void* workspace, size_t workspace_bytes, void* stream) {
-  int rc = validate(p, alpha, beta, workspace, workspace_bytes);
+  int rc = validate(p, alpha, beta, workspace, workspace_bytes, 0);
   if (rc) return rc;
   if (ntau < 0) return fail(KGEO_E_ARG, "ntau < 0%s (%lld)", "", ntau);
   if (ntau > 0 && p->npix > 0 && !tau_in) return fail(KGEO_E_ARG, "tau_in is NULL%s%lld");

@@ -119,7 +119,7 @@ def raytrace_ana_device(a, r_o, th_o, alpha, beta, ngeo, maxtaufrac=MAXTAUFRAC, 
                   ("tau_tot", torch.float64), ("status", torch.int32)):
         if k not in out:
             out[k] = torch.empty(npix, device=dev, dtype=dt)
-    wsb = _lib.workspace_bytes(npix)
+    wsb = _lib.workspace_bytes(npix, ngeo)
     ws = out.get("workspace")
     if ws is None or ws.numel() < wsb:
         ws = torch.empty(wsb + 256, device=dev, dtype=torch.uint8)

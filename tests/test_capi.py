@@ -35,13 +35,25 @@ def test_library_exports_every_declared_symbol():
 
 def test_load_and_version():
     lib = _lib.load()
-    assert lib.kgeo_abi_version() == 1
+    assert lib.kgeo_abi_version() == 2
     assert _lib.workspace_bytes(1000) == 256 + 1000 * _plan_slots() * 8
+
+
+def test_workspace_holds_the_walk_tables():
+    """On a linspace grid of ngeo > 2 samples the workspace adds the record
+    tables of the anchor pass (theta: ceil(ngeo/32) + 2 records of 6 fields;
+    radial: ceil(ngeo/16) + 2 records of 13 fields) and the fixup queue (one
+    8-byte item per 16-sample segment), per pixel."""
+    n = 1000
+    plan = _lib.workspace_bytes(n)
+    assert _lib.workspace_bytes(n, 2) == plan
+    assert _lib.workspace_bytes(n, 500) == plan + n * 8 * (18 * 6 + 34 * 13 + 32)
+    assert _lib.workspace_bytes(n, 33) == plan + n * 8 * (4 * 6 + 5 * 13 + 3)
 
 
 def _plan_slots():
     lib = _lib.load()
-    return (lib.kgeo_workspace_bytes(1) - 256) // 8
+    return (lib.kgeo_workspace_bytes(1, 0) - 256) // 8
 
 
 def test_fill_params_matches_numpy_expressions():

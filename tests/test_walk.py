@@ -125,3 +125,34 @@ def test_walking_loops_match_direct_evaluation(case):
         # an order of magnitude inside the 1e-10 parity bar
         assert e[~ill].max() < 2e-11, (k, float(e[~ill].max()))
         assert np.all(e[ill] < 100 * sens[ill] + 2e-11), k
+
+
+PASS_CASES = [
+    (0.94, 17.0, 8.0, 40, 500, 1),
+    (0.94, 17.0, 8.0, 40, 500, 3),
+    (-0.6, 70.0, 7.0, 30, 37, 1),
+    (0.5, 140.0, 6.0, 24, 50, 2),
+    (5.0e-5, 45.0, 8.0, 24, 80, 1),
+    (0.3, 60.0, 5.0, 12, 4, 1),
+]
+
+
+@pytest.mark.parametrize("case", PASS_CASES, ids=lambda c: f"a{c[0]}_th{c[1]}_G{c[4]}_ch{c[5]}")
+def test_three_passes_match_one_pass(case):
+    """The anchor / walk / fixup passes (the kernels' path on the linspace
+    grid) against one combined loop per coordinate: the same arithmetic in
+    another order, so the results agree to the FMA contractions the compiler
+    picks per function (~1e-15; bit-identical in a build without
+    contraction), with the same status bits; the fixup queue is exercised."""
+    a, thd, w, n, G, chunks = case
+    A, B = grid(w, n)
+    th = np.deg2rad(thd)
+    host = Host()
+    three = host.raytrace_loops(a, 1.0e4, th, A, B, G, chunks=chunks, passes=3)
+    one = host.raytrace_loops(a, 1.0e4, th, A, B, G, chunks=chunks, passes=1)
+    assert np.array_equal(three["status"], one["status"])
+    assert three["nfix"].sum() > 0
+    for k in ("tau", "t", "r", "th", "ph", "sig"):
+        assert np.array_equal(np.isnan(three[k]), np.isnan(one[k])), k
+        e = np.nan_to_num(errors(three[k], one[k], k))
+        assert e.max() < 1e-12, (k, float(e.max()))
