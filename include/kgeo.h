@@ -70,13 +70,10 @@ void kgeo_fill_params(kgeo_ray_params* p);
 
 /* Bytes of device workspace for npix pixels sampled on a linspace grid of
  * ngeo Mino times: 256 bytes of flags and counters, the per-pixel plan and,
- * for ngeo > 2, the walk's record tables and fixup queue (about 4.6 kB per
+ * for ngeo > 2, the walk's record tables and fixup queue (about 5.1 kB per
  * pixel at ngeo = 500).  Pass ngeo = 0 for kgeo_plan alone and for
  * caller-given Mino times (kgeo_coords_at_tau, kgeo_sample with tau_in).
- * Environment: KGEO_CHUNKS=<n> fixes the sample chunks per pixel;
- * KGEO_ONE_PASS=1 evaluates the grid in one combined pass per coordinate
- * instead of the anchor / walk / fixup passes (same results to rounding;
- * for comparisons). */
+ * Environment: KGEO_CHUNKS=<n> fixes the sample chunks per pixel. */
 size_t kgeo_workspace_bytes(int64_t npix, int64_t ngeo);
 
 /*

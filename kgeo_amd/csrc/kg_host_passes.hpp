@@ -1,6 +1,6 @@
-// The anchor / walk / fixup passes of the sample kernels (kg_sample.hpp,
+// The head / far / walk / fixup passes of the sample kernels (kg_sample.hpp,
 // kgeo_raytrace.hip) run on the CPU over all pixels, in the kernels' order:
-// theta anchors, theta walk, theta fixups, then the same for r.  Used by the
+// theta heads, long steps, walk, fixups, then the same for r.  Used by the
 // CPU test build (T = double) and the FLOP counter (T = its counting type).
 
 #pragma once
@@ -28,24 +28,29 @@ inline void unpack_host_item(uint64_t e, int64_t& i, int64_t& j, bool& walked) {
 }
 
 // st [npix] accumulates the status bits; nfix[2] receives the theta and
-// radial queue lengths.
+// radial queue lengths, or -1 where a pass queued more items than the
+// kernels' queue holds (npix * ceil(ngeo / kFixupSpan)).  The host queue
+// itself is unbounded, so the results stay complete either way.
 template <bool DO_PHI_T, class T>
 void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, const SampleOut& o, int chunks,
                   uint32_t* st, int64_t* nfix) {
   const int64_t npix = o.npix, G = o.ngeo;
   std::vector<double> trec(static_cast<size_t>(theta_records(G)) * kThFields * npix);
   std::vector<double> rrec(static_cast<size_t>(radial_records(G)) * kRFields * npix);
-  const Records rth{trec.data(), npix}, rr{rrec.data(), npix};
+  const Records rth{trec.data(), npix, 0}, rr{rrec.data(), npix, 0};
   std::vector<uint64_t> items;
   HostPush push{&items};
+  const int64_t nblk = anchor_blocks(G), nhead = anchor_heads(G);
   for (int64_t i = 0; i < npix; ++i)
-    for (int64_t rec = 0; rec < theta_records(G); ++rec)
-      st[i] |= theta_anchor_pixel<DO_PHI_T, T>(plan, npix, i, gflags, o, i, rec, rth);
+    for (int64_t hd = 0; hd < nhead; ++hd) st[i] |= theta_head_pixel<DO_PHI_T, T>(plan, npix, i, gflags, o, i, hd, rth);
+  for (int64_t i = 0; i < npix; ++i)
+    for (int64_t b = 0; b < nblk; ++b) st[i] |= theta_far_pixel<DO_PHI_T, T>(plan, npix, i, gflags, o, i, b, rth);
   for (int64_t i = 0; i < npix; ++i)
     for (int ch = 0; ch < chunks; ++ch)
       st[i] |= theta_walk_pixel<DO_PHI_T, HostPush, T>(plan, npix, i, gflags, o, i, chunk_start(G, ch, chunks),
                                                         chunk_start(G, ch + 1, chunks), rth, push);
-  nfix[0] = static_cast<int64_t>(items.size());
+  const size_t cap = static_cast<size_t>(npix) * static_cast<size_t>(seg_count(G, kFixupSpan));
+  nfix[0] = items.size() <= cap ? static_cast<int64_t>(items.size()) : -1;
   for (uint64_t e : items) {
     int64_t i, j;
     bool w;
@@ -54,13 +59,15 @@ void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, c
   }
   items.clear();
   for (int64_t i = 0; i < npix; ++i)
-    for (int64_t rec = 0; rec < radial_records(G); ++rec)
-      st[i] |= radial_anchor_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, rec, rr);
+    for (int64_t hd = 0; hd < nhead; ++hd)
+      st[i] |= radial_head_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, hd, rr);
+  for (int64_t i = 0; i < npix; ++i)
+    for (int64_t b = 0; b < nblk; ++b) st[i] |= radial_far_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, b, rr);
   for (int64_t i = 0; i < npix; ++i)
     for (int ch = 0; ch < chunks; ++ch)
       st[i] |= radial_walk_pixel<DO_PHI_T, HostPush, T>(c, plan, npix, i, gflags, o, i, chunk_start(G, ch, chunks),
                                                          chunk_start(G, ch + 1, chunks), rr, push);
-  nfix[1] = static_cast<int64_t>(items.size());
+  nfix[1] = items.size() <= cap ? static_cast<int64_t>(items.size()) : -1;
   for (uint64_t e : items) {
     int64_t i, j;
     bool w;

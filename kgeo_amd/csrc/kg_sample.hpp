@@ -165,6 +165,35 @@ KG_INLINE bool theta_loop_walk(const SampleOut& o, const ThetaConsts<T>& tc, con
   return bad;
 }
 
+// A direct block log entering the walk: next to its singularity its rounding
+// error (~eps e^|lg|) would be carried along the segment, so it enters as
+// NaN and the next step is evaluated directly.
+template <class T> KG_INLINE T block_log_enter(const T& lg) {
+  return kfabs(val(lg)) <= kLogGuard ? lg : T(NAN);
+}
+
+// One walked step of the case 3 / 4 block logs from state a to state b;
+// false (lb then unspecified) where a log leaves [-kLogGuard, kLogGuard].
+template <class T, int CASE>
+KG_INLINE bool radial_block_step(const RadialConsts<T>& rc, const WalkState<T, 3>& a, const WalkState<T, 3>& b,
+                                 T (&lb)[3]) {
+  T a0[3], b0[3], a1[3], b1[3], den[3], iden[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    block_log_args<T, CASE>(rc, q, a.s, a.d, a0[q], b0[q]);
+    block_log_args<T, CASE>(rc, q, b.s, b.d, a1[q], b1[q]);
+    den[q] = walk_log_den(a0[q], b0[q], a1[q], b1[q]);
+  }
+  shared_reciprocals<T, 3>(den, iden);
+  bool ok = true;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    lb[q] = walk_log(lb[q], a0[q], b0[q], a1[q], b1[q], iden[q]);
+    ok = ok && kfabs(val(lb[q])) <= kLogGuard;
+  }
+  return ok;
+}
+
 // Radial samples j0..j1-1 on the linspace grid, walking X_j = U0 - Xs j h.
 template <class T, int CASE, bool DO_PHI_T>
 KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
@@ -203,21 +232,12 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
     bool direct = (j == j0) || (j % kAnchor == 0) || (j == o.ngeo - 1);
     if (!direct) {
       WalkState<T, 3> nx;
-      if (walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
-        if (kBlocks) {
-          T a0[3], b0[3], a1[3], b1[3], den[3], iden[3];
+      T lb1[3] = {Lb[0], Lb[1], Lb[2]};
+      if (walk_step<T, 3, DO_PHI_T>(st, w, nx) && (!kBlocks || radial_block_step<T, CASE>(rc, st, nx, lb1))) {
 #pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            block_log_args<T, CASE>(rc, q, st.s, st.d, a0[q], b0[q]);
-            block_log_args<T, CASE>(rc, q, nx.s, nx.d, a1[q], b1[q]);
-            den[q] = walk_log_den(a0[q], b0[q], a1[q], b1[q]);
-          }
-          shared_reciprocals<T, 3>(den, iden);
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            Lb[q] = walk_log(Lb[q], a0[q], b0[q], a1[q], b1[q], iden[q]);
-            rs.lg[q] = Lb[q];
-          }
+        for (int q = 0; q < 3; ++q) {
+          Lb[q] = lb1[q];
+          rs.lg[q] = lb1[q];
         }
         st = nx;
         rs.s = st.s;
@@ -240,7 +260,7 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         if (DO_PHI_T) walk_enter(w, q, rs.Pi[q], X, rs.s, rs.c, rs.d, st);
-        if (kBlocks) Lb[q] = rs.lg[q];
+        if (kBlocks) Lb[q] = block_log_enter(rs.lg[q]);
       }
     }
     RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
@@ -271,7 +291,9 @@ template <class T> KG_INLINE uint32_t loop_status(bool bad, const JacobiLadder<T
 }
 
 // Theta part of samples j0..j1-1 of output pixel i, whose plan column is p.
-template <bool DO_PHI_T, class T = double>
+// WALK = false: every sample direct (the kernels for caller-given Mino times
+// and grids of at most two samples, which cannot walk).
+template <bool DO_PHI_T, class T = double, bool WALK = true>
 KG_INLINE uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
                                const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
   ThetaConsts<T> tc;
@@ -286,7 +308,7 @@ KG_INLINE uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, 
   const int k = (eta0 && !any_eta_pos) ? THK_NONE : kind;
   JacobiLadder<T> lad;
   jacobi_ladder(tc.v[TH_mp], lad);
-  const bool walk = walk_ok(o, any_step_zero);
+  const bool walk = WALK && walk_ok(o, any_step_zero);
   bool bad;
   switch (k) {
     case THK_ORD:
@@ -304,7 +326,7 @@ KG_INLINE uint32_t theta_pixel(const double* __restrict__ plan, int64_t stride, 
 }
 
 // Radial part (and the phi, t, sigma combination) of the same samples.
-template <bool DO_PHI_T, class T = double>
+template <bool DO_PHI_T, class T = double, bool WALK = true>
 KG_INLINE uint32_t radial_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride, int64_t p,
                                 uint32_t gflags, const SampleOut& o, int64_t i, int64_t j0, int64_t j1) {
   RadialConsts<T> rc;
@@ -317,7 +339,7 @@ KG_INLINE uint32_t radial_pixel(const RayScalars<T>& c, const double* __restrict
   const CombineConsts<T> cmb = combine_consts(c, lam);
   JacobiLadder<T> lad;
   jacobi_ladder(rc.v[R_k], lad);
-  const bool walk = walk_ok(o, any_step_zero) && lad.n >= 0;
+  const bool walk = WALK && walk_ok(o, any_step_zero) && lad.n >= 0;
   bool bad;
   switch (rc.rcase) {
     case 4:
@@ -337,40 +359,71 @@ KG_INLINE uint32_t radial_pixel(const RayScalars<T>& c, const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// The walk in three passes.  Each is its own kernel, so the walking loop is
-// not held to the register budget of the direct evaluation (Landen ladder,
-// Carlson duplication for three characteristics), which it needs only at a
-// segment's first sample:
-//   anchor  one lane per (pixel, record): the direct evaluation at one
-//           segment's first sample, at the last grid sample, or (record 0) at
-//           the step h itself; stores the walk state in the record table and,
-//           for a sample, its outputs;
+// The walk in passes, each its own kernel, so the walking loop is not held
+// to the register budget of the direct evaluation (Landen ladder, Carlson
+// duplication for three characteristics), which it needs only where it
+// restarts:
+//   heads   one lane per (pixel, head): the direct evaluation at the grid
+//           step h and at the segment length L h (records 0 and 1), at the
+//           last grid sample and at the first sample of each block of
+//           kAnchorBlock samples; stores the walk state in the record table
+//           and, for a sample, its outputs;
+//   far     one lane per (pixel, block): the block's later segment starts,
+//           each one addition step of L samples from the previous (record 1,
+//           R_C in closed form where the series does not reach); a refused
+//           step marks the block's remaining records unreachable;
 //   walk    one lane per (pixel, sample chunk): every other sample from the
-//           addition theorems alone, restarting from the records; a step that
-//           is not well conditioned queues its segment for the fixup pass and
-//           the lane resumes at the next segment;
-//   fixup   one lane per queued (pixel, segment): the combined loops above
-//           (direct evaluation wherever the walk cannot step) over the whole
-//           segment.  Pixels the walk does not cover (theta classes other
-//           than ordinary and vortical, radial m >= 1, numpy's zero-step
-//           grid, ngeo <= 2) queue all their segments.
+//           addition theorems alone, restarting from the records; a refused
+//           step (or an unreachable record) queues the rest of its segment
+//           (in ranges of at most kFixupSpan samples) for the fixup pass and
+//           the lane idles to the next segment;
+//   fixup   one lane per queued (pixel, range): the combined loops above
+//           (direct evaluation where the walk cannot step) over the range.  Pixels the walk does not cover
+//           (theta classes other than ordinary and vortical, radial m >= 1,
+//           numpy's zero-step grid, ngeo <= 2) queue all their segments.
 // A segment is kAnchorEveryLong samples (theta, radial case 2) or kAnchorEvery
-// samples (radial cases 3 and 4) starting at a multiple of its length, so the
-// direct evaluations fall on the samples the combined loops pick, whatever
-// the chunking.  Records are slot-major [record][field][stride]: record 0 the
-// step, record 1 + k the sample k * (theta: kAnchorEveryLong, radial:
+// samples (radial cases 3 and 4) starting at a multiple of its length; blocks
+// start at multiples of kAnchorBlock.  Both are fixed by the sample index, so
+// every sample is computed the same way whatever the chunking.  A sample is
+// at most kAnchorBlock / L - 1 long steps and L - 1 grid steps from a direct
+// evaluation.  Records are slot-major [record][field][stride]: 0 the step h,
+// 1 the step L h, 2 + k the sample k * (theta: kAnchorEveryLong, radial:
 // kAnchorEvery), the last record the sample ngeo - 1.
 constexpr int kThFields = 6;   // s, c, d, D, P, Lg
 constexpr int kRFields = 13;   // s, c, d, D, P[3], Lg[3], Lb[3] (walked R1/S1 logs)
+constexpr int64_t kAnchorBlock = 128;
 
 KG_INLINE int64_t seg_count(int64_t ngeo, int64_t len) { return (ngeo + len - 1) / len; }
-KG_INLINE int64_t theta_records(int64_t ngeo) { return seg_count(ngeo, kAnchorEveryLong) + 2; }
-KG_INLINE int64_t radial_records(int64_t ngeo) { return seg_count(ngeo, kAnchorEvery) + 2; }
 
+// A queued range ends at the next multiple of kFixupSpan (or the segment's
+// end): the fixup pass runs its ranges in parallel, and a range of refused
+// steps costs a direct evaluation per sample.  Each kFixupSpan window of a
+// pixel is queued at most once, so ceil(ngeo / kFixupSpan) items per pixel
+// bound the queue.
+constexpr int64_t kFixupSpan = 8;
+
+// Queue walked samples j .. min(segend, last) - 1 (the last grid sample is
+// the anchor pass's): from j to the next window start, then one item per
+// window.
+template <class Push>
+KG_INLINE void push_range(Push& push, int64_t i, int64_t j, int64_t segend, int64_t last) {
+  const int64_t e = segend < last ? segend : last;
+  push(i, j, true);
+  for (int64_t k = (j / kFixupSpan + 1) * kFixupSpan; k < e; k += kFixupSpan) push(i, k, true);
+}
+KG_INLINE int64_t theta_records(int64_t ngeo) { return seg_count(ngeo, kAnchorEveryLong) + 3; }
+KG_INLINE int64_t radial_records(int64_t ngeo) { return seg_count(ngeo, kAnchorEvery) + 3; }
+KG_INLINE int64_t anchor_blocks(int64_t ngeo) { return seg_count(ngeo, kAnchorBlock); }
+
+// Column col of the table is the plan column col0 + col - col0: a table of
+// one image of a batch starts at that image's first plan column col0.
 struct Records {
   double* p;
   int64_t stride;
-  KG_INLINE double* at(int64_t rec, int nf, int f, int64_t col) const { return p + (rec * nf + f) * stride + col; }
+  int64_t col0;
+  KG_INLINE double* at(int64_t rec, int nf, int f, int64_t col) const {
+    return p + (rec * nf + f) * stride + (col - col0);
+  }
 };
 
 template <class T, int NP>
@@ -402,38 +455,117 @@ KG_INLINE void record_load(const Records& rt, int64_t rec, int nf, int64_t col, 
   }
 }
 
-// Sample of record rec >= 1 whose segments are `len` samples.
-KG_INLINE int64_t record_sample(int64_t rec, int64_t ngeo, int64_t len) {
-  return (rec == seg_count(ngeo, len) + 1) ? ngeo - 1 : (rec - 1) * len;
-}
+// Record of the segment starting at sample js (segments of `len` samples in
+// the table's indexing), and of the last grid sample.
+KG_INLINE int64_t segment_record(int64_t js, int64_t len) { return 2 + js / len; }
+KG_INLINE int64_t last_record(int64_t ngeo, int64_t len) { return seg_count(ngeo, len) + 2; }
 
-// Theta record rec of output pixel i (direct evaluation).
+// Theta walk state from a direct evaluation at argument arg.
 template <class T, int KIND, bool DO_PHI_T>
-KG_INLINE bool theta_anchor(const SampleOut& o, const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, int64_t i,
-                            int64_t rec, double stop, const Records& rt, int64_t col) {
-  const T m = tc.v[TH_k];
-  WalkStep<T, 1> w;
-  const T n[1] = {tc.v[TH_n]};
-  walk_init(w, m, n);
-  const bool step = (rec == 0);
-  const int64_t j = step ? 0 : record_sample(rec, o.ngeo, kAnchorEveryLong);
-  const double tau = step ? 0.0 : mino_tau(j, o.ngeo, stop, false);
-  const T arg = step ? -tc.v[TH_w] * (stop / static_cast<double>(o.ngeo - 1)) : theta_arg(tc, T(tau));
-  ThState<T> ts;
+KG_INLINE void theta_direct(const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, const WalkStep<T, 1>& w,
+                            const T& arg, ThState<T>& ts, WalkState<T, 1>& st) {
   theta_state<T, KIND>(tc, lad, arg, DO_PHI_T, ts);
-  WalkState<T, 1> st;
   st.s = ts.s;
   st.c = ts.c;
   st.d = ts.d;
-  st.D = DO_PHI_T ? ts.EmF / m : T(0.0);
+  st.D = DO_PHI_T ? ts.EmF / tc.v[TH_k] : T(0.0);
   st.P[0] = T(0.0);
   st.Lg[0] = T(0.0);
   if (DO_PHI_T) walk_enter(w, 0, ts.Pi, arg, ts.s, ts.c, ts.d, st);
-  record_store<T, 1>(rt, rec, kThFields, col, st, static_cast<const T*>(nullptr));
-  if (step) return false;
+}
+
+template <class T>
+KG_INLINE void theta_init(const ThetaConsts<T>& tc, WalkStep<T, 1>& w) {
+  const T n[1] = {tc.v[TH_n]};
+  walk_init(w, tc.v[TH_k], n);
+}
+
+// Outputs of theta sample j from its state.
+template <class T, int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_emit(const SampleOut& o, const ThetaConsts<T>& tc, int64_t i, int64_t j, double tau,
+                          const T& arg, const ThState<T>& ts) {
   T th = T(0.0), Gph = T(0.0), Gt = T(0.0);
   theta_out<T, KIND>(tc, arg, ts, DO_PHI_T, th, Gph, Gt);
   return theta_store<T, DO_PHI_T>(o, j * o.npix + i, tau, th, Gph, Gt);
+}
+
+// Record store that marks a record the anchor pass could not reach (the
+// walk pass then queues its segment for the fixup pass).
+KG_INLINE void record_invalidate(const Records& rt, int64_t rec, int nf, int64_t col) {
+  *rt.at(rec, nf, 0, col) = NAN;
+}
+
+// Heads of the anchor pass: 0 the step h, 1 the step L h, 2 the last grid
+// sample, 2 + b (b >= 1) the first sample of block b - 1 (direct evaluation).
+KG_INLINE int64_t anchor_heads(int64_t ngeo) { return 3 + anchor_blocks(ngeo); }
+
+// Theta head hd of output pixel i (direct evaluation).
+template <class T, int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_head(const SampleOut& o, const ThetaConsts<T>& tc, const JacobiLadder<T>& lad, int64_t i,
+                          int64_t hd, double stop, const Records& rt, int64_t col) {
+  constexpr int64_t L = kAnchorEveryLong;
+  const int64_t last = o.ngeo - 1;
+  WalkStep<T, 1> w;
+  theta_init(tc, w);
+  ThState<T> ts;
+  WalkState<T, 1> st;
+  if (hd < 2) {
+    const double h = stop / static_cast<double>(o.ngeo - 1);
+    const T arg = -tc.v[TH_w] * (hd == 0 ? h : static_cast<double>(L) * h);
+    theta_direct<T, KIND, DO_PHI_T>(tc, lad, w, arg, ts, st);
+    record_store<T, 1>(rt, hd, kThFields, col, st, static_cast<const T*>(nullptr));
+    return false;
+  }
+  const int64_t j = (hd == 2) ? last : (hd - 3) * kAnchorBlock;
+  if (hd > 2 && j >= last) return false;  // the last sample is head 2
+  const double tau = mino_tau(j, o.ngeo, stop, false);
+  const T arg = theta_arg(tc, T(tau));
+  theta_direct<T, KIND, DO_PHI_T>(tc, lad, w, arg, ts, st);
+  record_store<T, 1>(rt, hd == 2 ? last_record(o.ngeo, L) : segment_record(j, L), kThFields, col, st,
+                     static_cast<const T*>(nullptr));
+  return theta_emit<T, KIND, DO_PHI_T>(o, tc, i, j, tau, arg, ts);
+}
+
+// Theta segment starts of block blk after its first sample, each one step of
+// L samples from the previous (addition theorems, no direct evaluation).  A
+// refused step leaves the block's remaining records marked unreachable.
+template <class T, int KIND, bool DO_PHI_T>
+KG_INLINE bool theta_far(const SampleOut& o, const ThetaConsts<T>& tc, int64_t i, int64_t blk, double stop,
+                         const Records& rt, int64_t col) {
+  constexpr int64_t L = kAnchorEveryLong;
+  const T m = tc.v[TH_k];
+  const int64_t last = o.ngeo - 1;
+  const int64_t jb = blk * kAnchorBlock;
+  const int64_t je = (jb + kAnchorBlock < last) ? jb + kAnchorBlock : last;
+  if (jb + L >= je) return false;
+  WalkStep<T, 1> w;  // the long step
+  theta_init(tc, w);
+  record_load<T, 1>(rt, 1, kThFields, col, w.b, static_cast<T*>(nullptr));
+  WalkState<T, 1> st;
+  record_load<T, 1>(rt, segment_record(jb, L), kThFields, col, st, static_cast<T*>(nullptr));
+  bool bad = false;
+  bool lost = false;
+  for (int64_t js = jb + L; js < je; js += L) {
+    WalkState<T, 1> nx;
+    lost = lost || !walk_step<T, 1, DO_PHI_T, true>(st, w, nx);
+    if (lost) {
+      record_invalidate(rt, segment_record(js, L), kThFields, col);
+      continue;
+    }
+    st = nx;
+    record_store<T, 1>(rt, segment_record(js, L), kThFields, col, st, static_cast<const T*>(nullptr));
+    const double tau = mino_tau(js, o.ngeo, stop, false);
+    const T arg = theta_arg(tc, T(tau));
+    ThState<T> ts;
+    ts.s = st.s;
+    ts.c = st.c;
+    ts.d = st.d;
+    ts.F = arg;
+    ts.EmF = m * st.D;
+    ts.Pi = DO_PHI_T ? walk_pi(w, 0, st, arg) : T(0.0);
+    bad |= theta_emit<T, KIND, DO_PHI_T>(o, tc, i, js, tau, arg, ts);
+  }
+  return bad;
 }
 
 // Theta samples j0..j1-1 walked from the records.  Where a step is refused,
@@ -446,8 +578,7 @@ KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t 
   constexpr int64_t L = kAnchorEveryLong;
   const T m = tc.v[TH_k];
   WalkStep<T, 1> w;
-  const T n[1] = {tc.v[TH_n]};
-  walk_init(w, m, n);
+  theta_init(tc, w);
   record_load<T, 1>(rt, 0, kThFields, col, w.b, static_cast<T*>(nullptr));
   const int64_t last = o.ngeo - 1;
   WalkState<T, 1> st;
@@ -456,14 +587,15 @@ KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t 
   for (int64_t j = j0; j < j1; ++j) {
     if (j == last) continue;  // written by the anchor pass
     if (j % L == 0) {
-      record_load<T, 1>(rt, 1 + j / L, kThFields, col, st, static_cast<T*>(nullptr));
-      queued = false;
+      record_load<T, 1>(rt, segment_record(j, L), kThFields, col, st, static_cast<T*>(nullptr));
+      queued = !kfinite(val(st.s));  // unreachable: the fixup pass takes the segment
+      if (queued) push_range(push, i, j, j + L, last);
       continue;
     }
     if (queued) continue;
     WalkState<T, 1> nx;
     if (!walk_step<T, 1, DO_PHI_T>(st, w, nx)) {
-      push(i, j, true);
+      push_range(push, i, j, j - j % L + L, last);
       queued = true;
       continue;
     }
@@ -485,52 +617,123 @@ KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t 
   return bad;
 }
 
-// Radial record rec of output pixel i (direct evaluation); case 2 keeps only
-// the records on its longer segments.
+template <class T, int CASE>
+KG_INLINE void radial_init(const RadialConsts<T>& rc, WalkStep<T, 3>& w) {
+  T n[3], pc[3];
+  radial_pi_params<T, CASE>(rc, n, pc);
+  walk_init(w, rc.v[R_k], n);
+}
+
+// Radial walk state (and block logs) from a direct evaluation at argument X
+// (dv: the Mino-time offset from the observer, or INFINITY for a plain step).
 template <class T, int CASE, bool DO_PHI_T>
-KG_INLINE bool radial_anchor(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
-                             const CombineConsts<T>& cmb, int64_t i, int64_t rec, double stop, const Records& rt,
-                             int64_t col) {
-  constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
+KG_INLINE void radial_direct(const RadialConsts<T>& rc, const JacobiLadder<T>& lad, const WalkStep<T, 3>& w,
+                             const T& X, const T& dv, RState<T>& rs, WalkState<T, 3>& st, T (&lb)[3]) {
   constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
-  const bool step = (rec == 0);
-  const int64_t j = step ? 0 : record_sample(rec, o.ngeo, kAnchorEvery);
-  if (!step && j != o.ngeo - 1 && j % L != 0) return false;
-  const T m = rc.v[R_k];
-  WalkStep<T, 3> w;
-  {
-    T n[3], pc[3];
-    radial_pi_params<T, CASE>(rc, n, pc);
-    walk_init(w, m, n);
-  }
-  const double tau = step ? 0.0 : mino_tau(j, o.ngeo, stop, false);
-  T X, dv;
-  if (step) {
-    X = -rc.v[R_Xs] * (stop / static_cast<double>(o.ngeo - 1));
-    dv = T(INFINITY);  // plain Jacobi functions at the step
-  } else {
-    dv = rc.v[R_Xs] * tau;
-    X = rc.v[R_U0] - dv;
-  }
-  RState<T> rs;
   radial_state<T, CASE>(rc, lad, X, dv, DO_PHI_T, rs);
-  WalkState<T, 3> st;
-  T lb[3];
   st.s = rs.s;
   st.c = rs.c;
   st.d = rs.d;
-  st.D = DO_PHI_T ? rs.EmF / m : T(0.0);
+  st.D = DO_PHI_T ? rs.EmF / rc.v[R_k] : T(0.0);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     st.P[q] = T(0.0);
     st.Lg[q] = T(0.0);
     if (DO_PHI_T) walk_enter(w, q, rs.Pi[q], X, rs.s, rs.c, rs.d, st);
-    lb[q] = (kBlocks && !step) ? rs.lg[q] : T(0.0);
+    lb[q] = kBlocks ? block_log_enter(rs.lg[q]) : T(0.0);
   }
-  record_store<T, 3>(rt, rec, kRFields, col, st, lb);
-  if (step) return false;
+}
+
+// Radial state for the outputs at argument X from a walked state.
+template <class T>
+KG_INLINE void radial_from_walk(const WalkStep<T, 3>& w, const WalkState<T, 3>& st, const T (&lb)[3], const T& X,
+                                const T& m, bool do_phi_t, RState<T>& rs) {
+  rs.s = st.s;
+  rs.c = st.c;
+  rs.d = st.d;
+  rs.F = X;
+  rs.EmF = m * st.D;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    rs.Pi[q] = do_phi_t ? walk_pi(w, q, st, X) : T(0.0);
+    rs.lg[q] = lb[q];
+  }
+}
+
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_emit(const SampleOut& o, const RadialConsts<T>& rc, const CombineConsts<T>& cmb, int64_t i,
+                           int64_t j, double tau, const RState<T>& rs) {
   RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
   return radial_store<T, DO_PHI_T>(o, cmb, j * o.npix + i, tau, ro);
+}
+
+// Radial head hd of output pixel i (direct evaluation; heads as for theta).
+// Records 0 and 1 hold plain Jacobi functions of the step argument.
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_head(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
+                           const CombineConsts<T>& cmb, int64_t i, int64_t hd, double stop, const Records& rt,
+                           int64_t col) {
+  constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
+  const int64_t last = o.ngeo - 1;
+  WalkStep<T, 3> w;
+  radial_init<T, CASE>(rc, w);
+  RState<T> rs;
+  WalkState<T, 3> st;
+  T lb[3];
+  if (hd < 2) {
+    const double h = stop / static_cast<double>(o.ngeo - 1);
+    const T X = -rc.v[R_Xs] * (hd == 0 ? h : static_cast<double>(L) * h);
+    radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, X, T(INFINITY), rs, st, lb);
+    lb[0] = lb[1] = lb[2] = T(0.0);  // no block logs at a step
+    record_store<T, 3>(rt, hd, kRFields, col, st, lb);
+    return false;
+  }
+  const int64_t j = (hd == 2) ? last : (hd - 3) * kAnchorBlock;
+  if (hd > 2 && j >= last) return false;  // the last sample is head 2
+  const double tau = mino_tau(j, o.ngeo, stop, false);
+  const T dv = rc.v[R_Xs] * tau;
+  const T X = rc.v[R_U0] - dv;
+  radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, X, dv, rs, st, lb);
+  record_store<T, 3>(rt, hd == 2 ? last_record(o.ngeo, kAnchorEvery) : segment_record(j, kAnchorEvery), kRFields,
+                     col, st, lb);
+  return radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, j, tau, rs);
+}
+
+// Radial segment starts of block blk after its first sample (as theta_far).
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_far(const SampleOut& o, const RadialConsts<T>& rc, const CombineConsts<T>& cmb, int64_t i,
+                          int64_t blk, double stop, const Records& rt, int64_t col) {
+  constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
+  constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
+  const T m = rc.v[R_k];
+  const int64_t last = o.ngeo - 1;
+  const int64_t jb = blk * kAnchorBlock;
+  const int64_t je = (jb + kAnchorBlock < last) ? jb + kAnchorBlock : last;
+  if (jb + L >= je) return false;
+  WalkStep<T, 3> w;  // the long step
+  radial_init<T, CASE>(rc, w);
+  record_load<T, 3>(rt, 1, kRFields, col, w.b, static_cast<T*>(nullptr));
+  WalkState<T, 3> st;
+  T lb[3] = {T(0.0), T(0.0), T(0.0)};
+  record_load<T, 3>(rt, segment_record(jb, kAnchorEvery), kRFields, col, st, kBlocks ? lb : static_cast<T*>(nullptr));
+  bool bad = false;
+  bool lost = false;
+  for (int64_t js = jb + L; js < je; js += L) {
+    WalkState<T, 3> nx;
+    lost = lost || !walk_step<T, 3, DO_PHI_T, true>(st, w, nx) ||
+           (kBlocks && !radial_block_step<T, CASE>(rc, st, nx, lb));
+    if (lost) {
+      record_invalidate(rt, segment_record(js, kAnchorEvery), kRFields, col);
+      continue;
+    }
+    st = nx;
+    record_store<T, 3>(rt, segment_record(js, kAnchorEvery), kRFields, col, st, lb);
+    const double tau = mino_tau(js, o.ngeo, stop, false);
+    RState<T> rs;
+    radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
+    bad |= radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, js, tau, rs);
+  }
+  return bad;
 }
 
 // Radial samples j0..j1-1 walked from the records.
@@ -541,65 +744,39 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
   constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
   const T m = rc.v[R_k];
   WalkStep<T, 3> w;
-  {
-    T n[3], pc[3];
-    radial_pi_params<T, CASE>(rc, n, pc);
-    walk_init(w, m, n);
-  }
+  radial_init<T, CASE>(rc, w);
   record_load<T, 3>(rt, 0, kRFields, col, w.b, static_cast<T*>(nullptr));
   const int64_t last = o.ngeo - 1;
   WalkState<T, 3> st;
-  T Lb[3];
+  T lb[3] = {T(0.0), T(0.0), T(0.0)};
   bool bad = false;
   bool queued = false;
   for (int64_t j = j0; j < j1; ++j) {
     if (j == last) continue;  // written by the anchor pass
     if (j % L == 0) {
-      record_load<T, 3>(rt, 1 + j / kAnchorEvery, kRFields, col, st, kBlocks ? Lb : static_cast<T*>(nullptr));
-      queued = false;
+      record_load<T, 3>(rt, segment_record(j, kAnchorEvery), kRFields, col, st,
+                        kBlocks ? lb : static_cast<T*>(nullptr));
+      queued = !kfinite(val(st.s));  // unreachable: the fixup pass takes the segment
+      if (queued) push_range(push, i, j, j + L, last);
       continue;
     }
     if (queued) continue;
     WalkState<T, 3> nx;
-    if (!walk_step<T, 3, DO_PHI_T>(st, w, nx)) {
-      push(i, j, true);
+    if (!walk_step<T, 3, DO_PHI_T>(st, w, nx) || (kBlocks && !radial_block_step<T, CASE>(rc, st, nx, lb))) {
+      push_range(push, i, j, j - j % L + L, last);
       queued = true;
       continue;
     }
-    RState<T> rs;
-    if (kBlocks) {
-      T a0[3], b0[3], a1[3], b1[3], den[3], iden[3];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        block_log_args<T, CASE>(rc, q, st.s, st.d, a0[q], b0[q]);
-        block_log_args<T, CASE>(rc, q, nx.s, nx.d, a1[q], b1[q]);
-        den[q] = walk_log_den(a0[q], b0[q], a1[q], b1[q]);
-      }
-      shared_reciprocals<T, 3>(den, iden);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        Lb[q] = walk_log(Lb[q], a0[q], b0[q], a1[q], b1[q], iden[q]);
-        rs.lg[q] = Lb[q];
-      }
-    }
     st = nx;
-    const int64_t idx = j * o.npix + i;
     const double tau = mino_tau(j, o.ngeo, stop, false);
-    const T X = rc.v[R_U0] - rc.v[R_Xs] * tau;
-    rs.s = st.s;
-    rs.c = st.c;
-    rs.d = st.d;
-    rs.F = X;
-    rs.EmF = m * st.D;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) rs.Pi[q] = DO_PHI_T ? walk_pi(w, q, st, X) : T(0.0);
-    RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
-    bad |= radial_store<T, DO_PHI_T>(o, cmb, idx, tau, ro);
+    RState<T> rs;
+    radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
+    bad |= radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, j, tau, rs);
   }
   return bad;
 }
 
-// Per-pixel dispatch of the three passes.  A theta pixel walks when its
+// Per-pixel dispatch of the passes.  A theta pixel walks when its
 // motion is ordinary or vortical; a radial pixel when its modulus is below 1.
 struct ThetaPick {
   int kind;
@@ -621,17 +798,30 @@ KG_INLINE ThetaPick theta_pick(const double* __restrict__ plan, int64_t stride, 
   return tp;
 }
 
+// Theta head hd of output pixel i (plan column p).
 template <bool DO_PHI_T, class T = double>
-KG_INLINE uint32_t theta_anchor_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
-                                      const SampleOut& o, int64_t i, int64_t rec, const Records& rt) {
+KG_INLINE uint32_t theta_head_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                    const SampleOut& o, int64_t i, int64_t hd, const Records& rt) {
   ThetaConsts<T> tc;
   const ThetaPick tp = theta_pick(plan, stride, p, gflags, o, tc);
   if (!tp.walk) return 0u;
   JacobiLadder<T> lad;
   jacobi_ladder(tc.v[TH_mp], lad);
-  const bool bad = (tp.kind == THK_ORD) ? theta_anchor<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, rec, tp.stop, rt, p)
-                                        : theta_anchor<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, rec, tp.stop, rt, p);
+  const bool bad = (tp.kind == THK_ORD) ? theta_head<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p)
+                                        : theta_head<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p);
   return loop_status(bad, lad);
+}
+
+// Theta segment starts of block blk of output pixel i after the block's head.
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t theta_far_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                   const SampleOut& o, int64_t i, int64_t blk, const Records& rt) {
+  ThetaConsts<T> tc;
+  const ThetaPick tp = theta_pick(plan, stride, p, gflags, o, tc);
+  if (!tp.walk) return 0u;
+  const bool bad = (tp.kind == THK_ORD) ? theta_far<T, THK_ORD, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p)
+                                        : theta_far<T, THK_VORT, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p);
+  return bad ? ST_NONFINITE : 0u;
 }
 
 template <bool DO_PHI_T, class Push, class T = double>
@@ -641,7 +831,7 @@ KG_INLINE uint32_t theta_walk_pixel(const double* __restrict__ plan, int64_t str
   ThetaConsts<T> tc;
   const ThetaPick tp = theta_pick(plan, stride, p, gflags, o, tc);
   if (!tp.walk) {
-    for (int64_t js = j0; js < j1; js += kAnchorEveryLong) push(i, js, false);
+    for (int64_t js = j0; js < j1; js += kFixupSpan) push(i, js, false);
     return 0u;
   }
   const bool bad = (tp.kind == THK_ORD)
@@ -650,21 +840,21 @@ KG_INLINE uint32_t theta_walk_pixel(const double* __restrict__ plan, int64_t str
   return bad ? ST_NONFINITE : 0u;
 }
 
-// End of the fixup range that starts at sample j inside a segment of `len`
-// samples: the segment's end, short of the last grid sample when the anchor
-// pass wrote it (walked: the radial store combines in place, so no sample may
-// be stored twice).
-KG_INLINE int64_t fixup_end(int64_t j, int64_t len, int64_t ngeo, bool walked) {
-  const int64_t je = (j / len + 1) * len;
+// End of the fixup range that starts at sample j: the next multiple of
+// kFixupSpan (which divides every segment length), short of the last grid
+// sample when the anchor pass wrote it (walked: the radial store combines in
+// place, so no sample may be stored twice).
+KG_INLINE int64_t fixup_end(int64_t j, int64_t ngeo, bool walked) {
+  const int64_t je = (j / kFixupSpan + 1) * kFixupSpan;
   const int64_t lim = walked ? ngeo - 1 : ngeo;
   return je < lim ? je : lim;
 }
 
-// Queued theta samples j.. to the end of their segment.
+// Queued theta samples j .. fixup_end.
 template <bool DO_PHI_T, class T = double>
 KG_INLINE uint32_t theta_fixup_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
                                      const SampleOut& o, int64_t i, int64_t j, bool walked) {
-  return theta_pixel<DO_PHI_T, T>(plan, stride, p, gflags, o, i, j, fixup_end(j, kAnchorEveryLong, o.ngeo, walked));
+  return theta_pixel<DO_PHI_T, T>(plan, stride, p, gflags, o, i, j, fixup_end(j, o.ngeo, walked));
 }
 
 template <class T>
@@ -682,36 +872,52 @@ KG_INLINE bool radial_walks(const SampleOut& o, uint32_t gflags, double m) {
   return walk_ok(o, (gflags & GF_ANY_STEP_ZERO) != 0) && m < 1.0;
 }
 
-// The anchor of one radial case (slots loaded inside the case, as in the walk).
-template <class T, int CASE, bool DO_PHI_T>
+// A head (FAR = false: direct evaluation) or the long steps of a block
+// (FAR = true) of one radial case, its slots loaded inside the case as in
+// the walk.
+template <class T, int CASE, bool DO_PHI_T, bool FAR>
 KG_INLINE uint32_t radial_anchor_case(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
-                                      int64_t p, const SampleOut& o, int64_t i, int64_t rec, const Records& rt) {
+                                      int64_t p, const SampleOut& o, int64_t i, int64_t k, const Records& rt) {
   RadialConsts<T> rc;
   T lam;
   double stop;
   radial_pick(plan, stride, p, rc, lam, stop);
   const CombineConsts<T> cmb = combine_consts(c, lam);
+  if (FAR) return radial_far<T, CASE, DO_PHI_T>(o, rc, cmb, i, k, stop, rt, p) ? ST_NONFINITE : 0u;
   JacobiLadder<T> lad;
   jacobi_ladder(rc.v[R_k], lad);
-  const bool bad = radial_anchor<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, rec, stop, rt, p);
+  const bool bad = radial_head<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, k, stop, rt, p);
   return loop_status(bad, lad);
 }
 
-template <bool DO_PHI_T, class T = double>
-KG_INLINE uint32_t radial_anchor_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
-                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t rec,
-                                       const Records& rt) {
+template <bool DO_PHI_T, bool FAR, class T = double>
+KG_INLINE uint32_t radial_anchor_dispatch(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                          int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t k,
+                                          const Records& rt) {
   const int rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
   const double m = plan[static_cast<int64_t>(PL_R + R_k) * stride + p];
   if (!radial_walks(o, gflags, m)) return 0u;
-  // case 2 keeps every other record (its segments are twice as long)
-  const int64_t j = (rec == 0) ? 0 : record_sample(rec, o.ngeo, kAnchorEvery);
-  if (rcase != 3 && rcase != 4 && rec != 0 && j != o.ngeo - 1 && j % kAnchorEveryLong != 0) return 0u;
   switch (rcase) {
-    case 4: return radial_anchor_case<T, 4, DO_PHI_T>(c, plan, stride, p, o, i, rec, rt);
-    case 3: return radial_anchor_case<T, 3, DO_PHI_T>(c, plan, stride, p, o, i, rec, rt);
-    default: return radial_anchor_case<T, 2, DO_PHI_T>(c, plan, stride, p, o, i, rec, rt);
+    case 4: return radial_anchor_case<T, 4, DO_PHI_T, FAR>(c, plan, stride, p, o, i, k, rt);
+    case 3: return radial_anchor_case<T, 3, DO_PHI_T, FAR>(c, plan, stride, p, o, i, k, rt);
+    default: return radial_anchor_case<T, 2, DO_PHI_T, FAR>(c, plan, stride, p, o, i, k, rt);
   }
+}
+
+// Radial head hd of output pixel i (plan column p).
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_head_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                     int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t hd,
+                                     const Records& rt) {
+  return radial_anchor_dispatch<DO_PHI_T, false, T>(c, plan, stride, p, gflags, o, i, hd, rt);
+}
+
+// Radial segment starts of block blk of output pixel i after the block's head.
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_far_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                    int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t blk,
+                                    const Records& rt) {
+  return radial_anchor_dispatch<DO_PHI_T, true, T>(c, plan, stride, p, gflags, o, i, blk, rt);
 }
 
 // The walk of one radial case: its plan slots are loaded here, inside the
@@ -735,8 +941,7 @@ KG_INLINE uint32_t radial_walk_pixel(const RayScalars<T>& c, const double* __res
   const int rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
   const double m = plan[static_cast<int64_t>(PL_R + R_k) * stride + p];
   if (!radial_walks(o, gflags, m)) {
-    const int len = (rcase == 3 || rcase == 4) ? kAnchorEvery : kAnchorEveryLong;
-    for (int64_t js = j0; js < j1; js += len) push(i, js, false);
+    for (int64_t js = j0; js < j1; js += kFixupSpan) push(i, js, false);
     return 0u;
   }
   bool bad;
@@ -748,14 +953,12 @@ KG_INLINE uint32_t radial_walk_pixel(const RayScalars<T>& c, const double* __res
   return bad ? ST_NONFINITE : 0u;
 }
 
-// Queued radial samples j.. to the end of their segment.
+// Queued radial samples j .. fixup_end.
 template <bool DO_PHI_T, class T = double>
 KG_INLINE uint32_t radial_fixup_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j,
                                       bool walked) {
-  const int64_t rcase = static_cast<int64_t>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
-  const int64_t len = (rcase == 3 || rcase == 4) ? kAnchorEvery : kAnchorEveryLong;
-  return radial_pixel<DO_PHI_T, T>(c, plan, stride, p, gflags, o, i, j, fixup_end(j, len, o.ngeo, walked));
+  return radial_pixel<DO_PHI_T, T>(c, plan, stride, p, gflags, o, i, j, fixup_end(j, o.ngeo, walked));
 }
 
 // Per-pixel part of the plan kernels: the plan column, labels and the two

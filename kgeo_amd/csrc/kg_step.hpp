@@ -35,6 +35,11 @@
 namespace kg {
 
 constexpr double kStepZ2 = 0.04;
+constexpr double kPoleGuard = 1.0e-4;
+// |ln|(1 + x)/(1 - x)|| above kLogGuard (|x| within ~1e-4 of 1) means a
+// walked log term sits next to its singularity, where the terms that cancel
+// in the outputs are each amplified by ~e^|ln| / 2: such steps go direct.
+constexpr double kLogGuard = 10.0;
 constexpr int kAnchorEvery = 16;       // radial cases 3 and 4
 constexpr int kAnchorEveryLong = 32;   // theta and radial case 2
 
@@ -76,6 +81,7 @@ template <class T, int NP> struct WalkStep {
   T n[NP], w2[NP];  // the walked characteristic and n (m - n)(1 - n)
   T pt[NP];         // sqrt((n - 1)(1 - m/n)) of a transformed integral, else 0
   T ipt2[NP];       // 1 / (2 pt)
+  T cw[NP], ncw[NP];  // sqrt(-w2) and n / sqrt(-w2) where w2 < 0, else 0
   bool tr[NP];
 };
 
@@ -109,20 +115,28 @@ KG_INLINE void walk_init(WalkStep<T, NP>& w, const T& m, const T (&n)[NP]) {
     w.ipt2[i] = w.tr[i] ? 0.5 / pt : T(0.0);
     w.n[i] = nw;
     w.w2[i] = nw * (m - nw) * (1.0 - nw);
+    const bool circ = val(w.w2[i]) < 0.0;
+    w.cw[i] = circ ? ksqrt(-w.w2[i]) : T(0.0);
+    w.ncw[i] = circ ? nw / w.cw[i] : T(0.0);
   }
 }
 
 // Direct entry into the walk at argument u: from Pi(n; am u), set the walked
-// integral (and Lg) of slot i.
+// integral (and Lg) of slot i.  Next to the pole of the integrand the direct
+// Pi(n) carries a rounding error of order eps / (1 - n sn^2), which the walk
+// would carry to every later sample of the segment: there the walked value is
+// set to NaN, so the next step is refused and evaluated directly too.
 template <class T, int NP>
 KG_INLINE void walk_enter(const WalkStep<T, NP>& w, int i, const T& Pn, const T& u, const T& sn, const T& cn,
                           const T& dn, WalkState<T, NP>& st) {
   if (w.tr[i]) {
     st.Lg[i] = log_ratio(w.pt[i] * sn, cn * dn);
     st.P[i] = -Pn + u + st.Lg[i] * w.ipt2[i];
+    if (!(kfabs(val(st.Lg[i])) <= kLogGuard)) st.P[i] = T(NAN);
   } else {
     st.Lg[i] = T(0.0);
     st.P[i] = Pn;
+    if (!(kfabs(val(1.0 - w.n[i] * (sn * sn))) >= kPoleGuard)) st.P[i] = T(NAN);
   }
 }
 
@@ -168,9 +182,22 @@ template <class T> KG_INLINE T walk_log_den(const T& a0, const T& b0, const T& a
   return b0 * b1 - a0 * a1;
 }
 
+// Steps of many samples at once (the anchor pass's step between segment
+// starts) take the Pi increment in closed form beyond the series' range.
+// With c = sqrt(|w2|) it is
+//   w2 < 0:  (n / c) atan2(c q, p)      (arctan form, any step),
+//   w2 > 0:  (n / c) artanh(c q / p)    (for z^2 up to kFarZ2).
+// w2 < 0 only for n < 1, where 1 - n sn^2 > 0 throughout, and then q = 0
+// forces p > 0 (q = 0 means sn u, sn v or sn(u+v) vanishes, and p reduces
+// to 1 - n sn^2 of the other end): the atan2 branch cut (q = 0, p < 0) is
+// never met, so the arctan form is continuous through p = 0 and needs no
+// sign condition on p.
+constexpr double kFarZ2 = 0.5;
+
 // One step a -> a + du.  Returns false (leaving `out` unspecified) when the
-// step is not well conditioned; the caller then evaluates directly.
-template <class T, int NP, bool INTEGRALS>
+// step is not well conditioned; the caller then evaluates directly.  FAR
+// admits z^2 < kFarZ2 (closed-form R_C) instead of |z^2| <= kStepZ2.
+template <class T, int NP, bool INTEGRALS, bool FAR = false>
 KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out) {
   const WalkState<T, NP>& b = w.b;
   const T ss = a.s * b.s;
@@ -205,9 +232,19 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
     const T& ip = ipv[i];
     const T nq = w.n[i] * q * ip;
     const T z2 = w.w2[i] * (q * ip) * (q * ip);
-    ok = ok && ((val(r0) > 0.0) == (val(r1) > 0.0)) && ((val(p) > 0.0) == (val(r1) > 0.0)) &&
-         (kfabs(val(z2)) <= kStepZ2);
-    out.P[i] = a.P[i] + b.P[i] + nq * rc_one_series(z2);
+    const bool series = kfabs(val(z2)) <= kStepZ2;
+    const bool psign = (val(p) > 0.0) == (val(r1) > 0.0);
+    // next to the integrand's pole (1 - n sn^2 -> 0: n > 1, or n -> 1 at
+    // sn -> 1) 1 - n sn^2 carries an absolute rounding error of a few ulp,
+    // which each step would add to Pi relative to 1 / (1 - n sn^2)
+    ok = ok && kfabs(val(r1)) >= kPoleGuard;
+    const bool circ = FAR && val(w.w2[i]) < 0.0 && !(series && psign);
+    ok = ok && ((val(r0) > 0.0) == (val(r1) > 0.0)) &&
+         (circ || (psign && (series || (FAR && val(z2) <= kFarZ2))));
+    T corr;
+    if (circ) corr = w.ncw[i] * katan2(w.cw[i] * q, p);
+    else corr = nq * ((!FAR || series) ? rc_one_series(z2) : rc_1pe(-z2));
+    out.P[i] = a.P[i] + b.P[i] + corr;
     ok = ok && kfinite(val(out.P[i]));
   }
   // the transformed integrals' log terms, one shared reciprocal
@@ -227,7 +264,7 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
       } else {
         out.Lg[i] = log_ratio(w.pt[i] * S, C * Dn);
       }
-      ok = ok && kfinite(val(out.Lg[i]));
+      ok = ok && kfabs(val(out.Lg[i])) <= kLogGuard;  // false for NaN
     } else {
       out.Lg[i] = T(0.0);
     }

@@ -1,22 +1,23 @@
 // Batched survey: many raytrace_ana images, each with its own spin,
-// inclination, resolution and ngeo, in one persistent launch that is
+// inclination, resolution and ngeo, in persistent launches that are
 // captured once in a HIP graph and replayed.
 //
-//   kg_batch_clear       zero the per-image flag words, the per-image
-//                        completed-item counters and the work counter
+//   kg_batch_clear       zero the per-image flag words and item counters,
+//                        the pass counters and the fixup-queue counters
 //   kg_batch_setup       one lane per pixel slot of every image: the same
 //                        per-pixel plan and labels as kg_setup.  Each image's
 //                        slots are padded to a multiple of 64, so a wave never
 //                        straddles two images and the image-wide predicates
 //                        (any eta > 0, any zero grid step) are per-image
 //                        wave ballots.
-//   kg_batch_persistent  a fixed grid of waves; each wave takes work items
-//                        (image, 64-pixel tile, Mino-time chunk) from one
-//                        atomic counter until the queue is empty and runs the
-//                        theta loop then the radial loop of that item
-//                        (kg_sample.hpp, the code the plain kernels run).
-//                        Every wave leaves once the counter passes the item
-//                        total, so the grid always drains.
+//   kg_batch_pass<P>     for theta then r, the heads, far and walk passes of
+//                        the plain kernels (kg_sample.hpp), each a fixed grid
+//                        of waves taking work items (image, 64-pixel tile,
+//                        head / block / Mino-time chunk) from its own atomic
+//                        counter until the pass is empty; every wave leaves
+//                        once the counter passes the pass's item total, so
+//                        the grid always drains;
+//   kg_batch_fixup<R>    the queued ranges of all images (grid-stride).
 // Outputs go to caller arenas: image k's [ngeo_k][npix_k] block of r, theta,
 // phi, t starts at element out_off[k] (sum of npix*ngeo of the images before
 // it); per-pixel inputs and labels are concatenated at pix_off[k].
@@ -56,14 +57,23 @@ constexpr int64_t kMaxItems = (1LL << 31) - (1LL << 24);  // leaves room for one
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+// pass kinds and their per-tile work units
+enum : int { K_HEAD = 0, K_FAR = 1, K_WALK = 2, K_N = 3 };
+// pass counters: theta heads, far, walk, then r heads, far, walk; then the
+// theta and radial fixup-queue counters
+constexpr int kPassCounters = 2 * K_N;
+constexpr int kQueueCounter = kPassCounters;  // + 0 theta, + 1 r
+
 struct BatchImage {
   RayScalars<double> c;
   int64_t npix, ngeo;
   int64_t pix_off;   // first pixel in the concatenated per-pixel arrays
   int64_t slot_off;  // first plan column (64-aligned)
+  int64_t slots;     // padded pixel count
   int64_t out_off;   // first element of the image's [ngeo][npix] block
-  int64_t item_off;  // first work item
-  int32_t chunks;    // Mino-time chunks per pixel
+  int64_t item_off[K_N];  // first work item of each pass kind
+  int64_t th_rec, r_rec;  // first element of the image's record tables
+  int32_t units[K_N];     // heads, anchor blocks, Mino-time chunks per tile
   int32_t do_phi_t;
 };
 
@@ -71,13 +81,16 @@ struct BatchDev {
   const BatchImage* imgs;
   int nimg;
   int64_t total_slots;
-  uint32_t total_items;
+  uint32_t total_items[K_N];
   const double* alpha;
   const double* beta;
   double* plan;       // [PL_N][total_slots]
+  double* rec;        // per-image record tables
+  unsigned long long* queue;
+  unsigned long long qcap;
   uint32_t* flags;    // [nimg]
   uint32_t* done;     // [nimg] completed items
-  uint32_t* counter;
+  uint32_t* counter;  // [kPassCounters + 2]
   double *t, *r, *th, *ph;
   int8_t* rclass;
   int8_t* thclass;
@@ -86,12 +99,13 @@ struct BatchDev {
   uint32_t* status;
 };
 
-// last image k with key(k) <= key; every lane of a wave searches the same key
-__device__ inline int find_image(const BatchImage* imgs, int n, int64_t key, bool by_item) {
+// last image k with key(k) <= key (kind < 0: plan columns, else the first
+// item of that pass kind); every lane of a wave searches the same key
+__device__ inline int find_image(const BatchImage* imgs, int n, int64_t key, int kind) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const int64_t off = by_item ? imgs[mid].item_off : imgs[mid].slot_off;
+    const int64_t off = kind < 0 ? imgs[mid].slot_off : imgs[mid].item_off[kind];
     if (off <= key) lo = mid;
     else hi = mid - 1;
   }
@@ -103,14 +117,14 @@ __global__ void kg_batch_clear(BatchDev d) {
     d.flags[k] = 0u;
     d.done[k] = 0u;
   }
-  if (threadIdx.x == 0) d.counter[0] = 0u;
+  if (threadIdx.x < kPassCounters + 2) d.counter[threadIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void kg_batch_setup(BatchDev d) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   // total_slots is a multiple of 64: whole waves leave here together
   if (s >= d.total_slots) return;
-  const int im = find_image(d.imgs, d.nimg, s, false);
+  const int im = find_image(d.imgs, d.nimg, s, -1);
   const BatchImage& B = d.imgs[im];
   const int64_t i = s - B.slot_off;
   SetupOut so{false, false};
@@ -135,36 +149,110 @@ __device__ inline uint32_t fetch_item(uint32_t* counter, int lane) {
   return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(got)));
 }
 
-__global__ __launch_bounds__(kBlock) void kg_batch_persistent(BatchDev d) {
+// Fixup queue item of a batch: walked << 63 | image << 51 | sample << 31 | pixel
+// (images < 4096, ngeo <= 2^20, npix < 2^31: checked at plan time).
+struct BatchPush {
+  unsigned long long* items;
+  uint32_t* count;
+  unsigned long long cap;
+  unsigned long long image;
+  __device__ void operator()(int64_t i, int64_t j, bool walked) const {
+    const uint32_t s = atomicAdd(count, 1u);
+    if (s < cap)
+      items[s] = (walked ? (1ull << 63) : 0ull) | (image << 51) | (static_cast<unsigned long long>(j) << 31) |
+                 static_cast<unsigned long long>(i);
+  }
+};
+
+__device__ inline SampleOut image_out(const BatchDev& d, const BatchImage& B) {
+  const bool phi_t = B.do_phi_t != 0;
+  return SampleOut{nullptr, nullptr, phi_t ? d.t + B.out_off : nullptr, d.r + B.out_off, d.th + B.out_off,
+                   phi_t ? d.ph + B.out_off : nullptr, nullptr, B.npix, B.ngeo};
+}
+
+__device__ inline Records image_records(const BatchDev& d, const BatchImage& B, bool theta) {
+  return Records{d.rec + (theta ? B.th_rec : B.r_rec), B.slots, B.slot_off};
+}
+
+// One unit of pass KIND (theta if THETA) for pixel i of image B.
+template <int KIND, bool THETA, bool PHI_T>
+__device__ inline uint32_t batch_unit(const BatchDev& d, const BatchImage& B, int im, int64_t i, int64_t u) {
+  const int64_t s = B.slot_off + i;
+  const uint32_t fl = d.flags[im];
+  const SampleOut o = image_out(d, B);
+  const Records rt = image_records(d, B, THETA);
+  if (KIND == K_HEAD) {
+    return THETA ? theta_head_pixel<PHI_T>(d.plan, d.total_slots, s, fl, o, i, u, rt)
+                 : radial_head_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, u, rt);
+  }
+  if (KIND == K_FAR) {
+    return THETA ? theta_far_pixel<PHI_T>(d.plan, d.total_slots, s, fl, o, i, u, rt)
+                 : radial_far_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, u, rt);
+  }
+  const int64_t j0 = chunk_start(B.ngeo, u, B.units[K_WALK]);
+  const int64_t j1 = chunk_start(B.ngeo, u + 1, B.units[K_WALK]);
+  BatchPush push{d.queue, d.counter + kQueueCounter + (THETA ? 0 : 1), d.qcap, static_cast<unsigned long long>(im)};
+  return THETA ? theta_walk_pixel<PHI_T>(d.plan, d.total_slots, s, fl, o, i, j0, j1, rt, push)
+               : radial_walk_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, j0, j1, rt, push);
+}
+
+template <int KIND, bool THETA>
+__device__ inline void batch_pass(const BatchDev& d) {
   const int lane = __lane_id();
+  uint32_t* counter = d.counter + (THETA ? 0 : K_N) + KIND;
   for (;;) {
-    const uint32_t item = fetch_item(d.counter, lane);
-    if (item >= d.total_items) break;  // scalar branch: the whole wave leaves once the queue is empty
-    const int im = find_image(d.imgs, d.nimg, static_cast<int64_t>(item), true);
+    const uint32_t item = fetch_item(counter, lane);
+    if (item >= d.total_items[KIND]) break;  // scalar branch: the whole wave leaves once the pass is empty
+    const int im = find_image(d.imgs, d.nimg, static_cast<int64_t>(item), KIND);
     const BatchImage& B = d.imgs[im];
-    const int64_t li = static_cast<int64_t>(item) - B.item_off;
-    const int64_t tile = li / B.chunks;
-    const int64_t chunk = li % B.chunks;
-    const int64_t i = tile * kWave + lane;
+    const int64_t li = static_cast<int64_t>(item) - B.item_off[KIND];
+    const int64_t units = B.units[KIND];
+    const int64_t i = (li / units) * kWave + lane;
+    const int64_t u = li % units;
     if (i < B.npix) {
-      const int64_t s = B.slot_off + i;
-      const int64_t j0 = chunk_start(B.ngeo, chunk, B.chunks);
-      const int64_t j1 = chunk_start(B.ngeo, chunk + 1, B.chunks);
-      const uint32_t fl = d.flags[im];
-      const bool phi_t = B.do_phi_t != 0;
-      SampleOut o{nullptr, nullptr, phi_t ? d.t + B.out_off : nullptr, d.r + B.out_off, d.th + B.out_off,
-                  phi_t ? d.ph + B.out_off : nullptr, nullptr, B.npix, B.ngeo};
-      uint32_t st;
-      if (phi_t) {
-        st = theta_pixel<true>(d.plan, d.total_slots, s, fl, o, i, j0, j1);
-        st |= radial_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j0, j1);
-      } else {
-        st = theta_pixel<false>(d.plan, d.total_slots, s, fl, o, i, j0, j1);
-        st |= radial_pixel<false>(B.c, d.plan, d.total_slots, s, fl, o, i, j0, j1);
-      }
+      const uint32_t st = B.do_phi_t ? batch_unit<KIND, THETA, true>(d, B, im, i, u)
+                                     : batch_unit<KIND, THETA, false>(d, B, im, i, u);
       if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
     }
     atomicAdd(&d.done[im], lane == 0 ? 1u : 0u);
+  }
+}
+
+template <int KIND, bool THETA>
+__global__ __launch_bounds__(kBlock) void kg_batch_pass(BatchDev d) {
+  batch_pass<KIND, THETA>(d);
+}
+
+// the walk pass held to two waves per SIMD, as kg_radial_walk
+template <bool THETA>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void kg_batch_walk(BatchDev d) {
+  batch_pass<K_WALK, THETA>(d);
+}
+
+template <bool THETA>
+__global__ __launch_bounds__(kBlock) void kg_batch_fixup(BatchDev d) {
+  const unsigned long long c = d.counter[kQueueCounter + (THETA ? 0 : 1)];
+  const unsigned long long n = c < d.qcap ? c : d.qcap;
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+    const unsigned long long q = d.queue[e];
+    const int im = static_cast<int>((q >> 51) & 0xfffull);
+    const int64_t j = static_cast<int64_t>((q >> 31) & 0xfffffull);
+    const int64_t i = static_cast<int64_t>(q & 0x7fffffffull);
+    const bool walked = (q >> 63) != 0ull;
+    const BatchImage& B = d.imgs[im];
+    const int64_t s = B.slot_off + i;
+    const uint32_t fl = d.flags[im];
+    const SampleOut o = image_out(d, B);
+    uint32_t st;
+    if (THETA) {
+      st = B.do_phi_t ? theta_fixup_pixel<true>(d.plan, d.total_slots, s, fl, o, i, j, walked)
+                      : theta_fixup_pixel<false>(d.plan, d.total_slots, s, fl, o, i, j, walked);
+    } else {
+      st = B.do_phi_t ? radial_fixup_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked)
+                      : radial_fixup_pixel<false>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked);
+    }
+    if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
   }
 }
 
@@ -181,17 +269,46 @@ RayScalars<double> scalars_of(const kgeo_ray_params& p) {
 
 int check_images(const kgeo_ray_params* imgs, int32_t nimg) {
   if (!imgs || nimg <= 0) return bfail(KGEO_E_ARG, "no images (nimg = %lld)", nimg);
+  if (nimg > 4096) return bfail(KGEO_E_ARG, "more than 4096 images (%lld)", nimg);
   for (int32_t k = 0; k < nimg; ++k) {
     if (imgs[k].npix < 1) return bfail(KGEO_E_ARG, "image %lld has npix < 1", k);
+    if (imgs[k].npix >= (1LL << 31)) return bfail(KGEO_E_ARG, "image %lld has npix >= 2^31", k);
     if (imgs[k].ngeo < 1) return bfail(KGEO_E_ARG, "image %lld has ngeo < 1", k);
+    if (imgs[k].ngeo > (1LL << 20)) return bfail(KGEO_E_ARG, "image %lld has ngeo > 2^20", k);
   }
   return KGEO_OK;
 }
 
+int64_t slots_of(const kgeo_ray_params& p) { return (p.npix + kWave - 1) / kWave * kWave; }
+
 int64_t total_slots_of(const kgeo_ray_params* imgs, int32_t nimg) {
   int64_t s = 0;
-  for (int32_t k = 0; k < nimg; ++k) s += (imgs[k].npix + kWave - 1) / kWave * kWave;
+  for (int32_t k = 0; k < nimg; ++k) s += slots_of(imgs[k]);
   return s;
+}
+
+// Elements of image k's theta and radial record tables (a walked grid needs
+// ngeo > 2; smaller images keep none) and its fixup-queue share.
+int64_t theta_rec_elems(const kgeo_ray_params& p) {
+  return p.ngeo > 2 ? theta_records(p.ngeo) * kThFields * slots_of(p) : 0;
+}
+int64_t radial_rec_elems(const kgeo_ray_params& p) {
+  return p.ngeo > 2 ? radial_records(p.ngeo) * kRFields * slots_of(p) : 0;
+}
+int64_t queue_elems(const kgeo_ray_params& p) { return seg_count(p.ngeo, kFixupSpan) * slots_of(p); }
+
+struct BatchSizes {
+  int64_t slots, rec, queue;
+};
+
+BatchSizes sizes_of(const kgeo_ray_params* imgs, int32_t nimg) {
+  BatchSizes z{0, 0, 0};
+  for (int32_t k = 0; k < nimg; ++k) {
+    z.slots += slots_of(imgs[k]);
+    z.rec += theta_rec_elems(imgs[k]) + radial_rec_elems(imgs[k]);
+    z.queue += queue_elems(imgs[k]);
+  }
+  return z;
 }
 
 }  // namespace
@@ -209,9 +326,12 @@ const char* kgeo_batch_last_error(void) { return g_err_b; }
 
 size_t kgeo_batch_workspace_bytes(const kgeo_ray_params* imgs, int32_t nimg) {
   if (check_images(imgs, nimg) != KGEO_OK) return 0;
+  const BatchSizes z = sizes_of(imgs, nimg);
   return align256(sizeof(BatchImage) * static_cast<size_t>(nimg)) +
          2 * align256(sizeof(uint32_t) * static_cast<size_t>(nimg)) + 256 +
-         static_cast<size_t>(total_slots_of(imgs, nimg)) * PL_N * sizeof(double);
+         align256(static_cast<size_t>(z.slots) * PL_N * sizeof(double)) +
+         align256(static_cast<size_t>(z.rec) * sizeof(double)) +
+         static_cast<size_t>(z.queue) * sizeof(unsigned long long);
 }
 
 int kgeo_batch_layout(const kgeo_ray_params* imgs, int32_t nimg, int64_t* pix_off, int64_t* out_off) {
@@ -251,8 +371,12 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
 
   std::vector<BatchImage> host(nimg);
   std::vector<uint32_t> per(nimg);
-  int64_t pix = 0, slots = 0, outo = 0, items = 0;
+  int64_t pix = 0, slots = 0, outo = 0, rec = 0;
+  int64_t items[K_N] = {0, 0, 0};
   bool any_phi = false;
+  // walk items hold whole segments (chunk starts are multiples of kAnchorEveryLong)
+  const int64_t spi =
+      (static_cast<int64_t>(tn.samples_per_item) + kAnchorEveryLong - 1) / kAnchorEveryLong * kAnchorEveryLong;
   for (int32_t k = 0; k < nimg; ++k) {
     const kgeo_ray_params& p = imgs[k];
     BatchImage& B = host[k];
@@ -261,22 +385,29 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
     B.ngeo = p.ngeo;
     B.pix_off = pix;
     B.slot_off = slots;
+    B.slots = slots_of(p);
     B.out_off = outo;
-    B.item_off = items;
-    // items hold whole anchor intervals (chunk starts are multiples of kAnchorEveryLong)
-    const int64_t spi =
-        (static_cast<int64_t>(tn.samples_per_item) + kAnchorEveryLong - 1) / kAnchorEveryLong * kAnchorEveryLong;
-    const int64_t ch = (p.ngeo + spi - 1) / spi;
-    B.chunks = static_cast<int32_t>(ch);
+    B.units[K_HEAD] = static_cast<int32_t>(anchor_heads(p.ngeo));
+    B.units[K_FAR] = static_cast<int32_t>(anchor_blocks(p.ngeo));
+    B.units[K_WALK] = static_cast<int32_t>((p.ngeo + spi - 1) / spi);
+    B.th_rec = rec;
+    rec += theta_rec_elems(p);
+    B.r_rec = rec;
+    rec += radial_rec_elems(p);
     B.do_phi_t = (p.flags & KGEO_DO_PHI_T) ? 1 : 0;
     any_phi |= B.do_phi_t != 0;
-    const int64_t tiles = (p.npix + kWave - 1) / kWave;
+    const int64_t tiles = B.slots / kWave;
+    int64_t mine = 0;
+    for (int q = 0; q < K_N; ++q) {
+      B.item_off[q] = items[q];
+      items[q] += tiles * B.units[q];
+      mine += tiles * B.units[q];
+      if (items[q] > kMaxItems) return bfail(KGEO_E_ARG, "too many work items (%lld); raise samples_per_item", items[q]);
+    }
     pix += p.npix;
-    slots += tiles * kWave;
+    slots += B.slots;
     outo += p.npix * p.ngeo;
-    items += tiles * ch;
-    if (items > kMaxItems) return bfail(KGEO_E_ARG, "too many work items (%lld); raise samples_per_item", items);
-    per[k] = static_cast<uint32_t>(tiles * ch);
+    per[k] = static_cast<uint32_t>(2 * mine);  // theta and radial passes
   }
   if (any_phi && (!t || !ph)) return bfail(KGEO_E_ARG, "t and ph are required with KGEO_DO_PHI_T");
 
@@ -291,9 +422,14 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
   d.counter = reinterpret_cast<uint32_t*>(w);
   w += 256;
   d.plan = reinterpret_cast<double*>(w);
+  w += align256(static_cast<size_t>(slots) * PL_N * sizeof(double));
+  d.rec = reinterpret_cast<double*>(w);
+  w += align256(static_cast<size_t>(rec) * sizeof(double));
+  d.queue = reinterpret_cast<unsigned long long*>(w);
+  d.qcap = static_cast<unsigned long long>(sizes_of(imgs, nimg).queue);
   d.nimg = nimg;
   d.total_slots = slots;
-  d.total_items = static_cast<uint32_t>(items);
+  for (int q = 0; q < K_N; ++q) d.total_items[q] = static_cast<uint32_t>(items[q]);
   d.alpha = alpha;
   d.beta = beta;
   d.t = t;
@@ -332,7 +468,14 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
   }
   hipLaunchKernelGGL(kg_batch_clear, dim3(1), dim3(kBlock), 0, st, d);
   hipLaunchKernelGGL(kg_batch_setup, dim3(setup_blocks), dim3(kBlock), 0, st, d);
-  hipLaunchKernelGGL(kg_batch_persistent, dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL((kg_batch_pass<K_HEAD, true>), dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL((kg_batch_pass<K_FAR, true>), dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(kg_batch_walk<true>, dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(kg_batch_fixup<true>, dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL((kg_batch_pass<K_HEAD, false>), dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL((kg_batch_pass<K_FAR, false>), dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(kg_batch_walk<false>, dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(kg_batch_fixup<false>, dim3(blocks), dim3(kBlock), 0, st, d);
   const int lrc = blaunch("batch kernel launch (capture)");
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(st, &g);
@@ -358,7 +501,12 @@ int kgeo_batch_run(kgeo_batch* b, void* stream) {
   return KGEO_OK;
 }
 
-int64_t kgeo_batch_items(const kgeo_batch* b) { return b ? static_cast<int64_t>(b->d.total_items) : -1; }
+int64_t kgeo_batch_items(const kgeo_batch* b) {
+  if (!b) return -1;
+  int64_t n = 0;
+  for (int q = 0; q < K_N; ++q) n += 2 * static_cast<int64_t>(b->d.total_items[q]);
+  return n;
+}
 
 int kgeo_batch_progress(const kgeo_batch* b, uint32_t* done_items, uint32_t* total_items) {
   if (!b) return bfail(KGEO_E_ARG, "batch handle is NULL");

@@ -2,18 +2,27 @@
 // Kerr raytracer.
 //
 // Pipeline of one kgeo_raytrace_ana call, all on the caller's stream:
-//   kg_clear_flags   one wave: zero the image-wide flag word in the workspace
+//   kg_clear_flags   one wave: zero the flag page of the workspace
 //   kg_setup         one lane per pixel: lambda, eta, u_+-, roots, labels,
 //                    tau_tot, nmax_eq and the per-pixel constants of the
 //                    theta and radial integrals -> slot-major plan [PL_N][npix];
 //                    wave-ballot OR of the image-wide flags the reference's
 //                    numpy code decides globally (any eta > 0, any zero step)
-//   kg_theta,        one lane per (pixel, sample chunk): loads its plan slots
-//   kg_radial        once (coalesced), builds its Jacobi ladder, loops over
-//                    its Mino-time samples and stores r, theta, phi, t, sigma
-//                    at [j][npix] -- 64 lanes write 512 contiguous bytes.
-// The work is FP64 VALU (no MFMA): Carlson duplication, Landen ladders and
-// transcendentals per sample.
+// then, on the linspace grid, per coordinate (theta first: its G_phi, G_t
+// are parked in the phi and t outputs; the radial stores combine them):
+//   kg_*_heads       one lane per (pixel, head): direct evaluations (Landen
+//                    ladder, Carlson duplication) at the steps h and L h, the
+//                    last sample and each block's first sample -> records
+//   kg_*_far         one lane per (pixel, block): the block's other segment
+//                    starts by long addition steps -> records
+//   kg_*_walk        one lane per (pixel, sample chunk): every other sample
+//                    by the addition theorems from the records; 64 lanes
+//                    write 512 contiguous bytes per output and sample
+//   kg_*_fixup       one lane per queued range: direct evaluation where the
+//                    walk refused a step
+// (kg_sample.hpp).  Caller-given Mino times and grids of at most two samples
+// take kg_theta / kg_radial, one combined loop per coordinate.  The work is
+// FP64 VALU (no MFMA).
 
 #include <hip/hip_runtime.h>
 
@@ -98,9 +107,9 @@ __global__ __launch_bounds__(kBlock) void kg_setup(DevParams p, const double* __
 }
 
 // ---------------------------------------------------------------------------
-// Sample kernels: grid (pixel blocks, sample chunks).  kg_theta writes theta
-// and parks G_phi, G_t in ph, t; kg_radial writes r and combines phi, t,
-// sigma (kg_sample.hpp).
+// Direct sample kernels (caller-given Mino times, ngeo <= 2): grid (pixel
+// blocks, sample chunks).  kg_theta writes theta and parks G_phi, G_t in ph,
+// t; kg_radial writes r and combines phi, t, sigma (kg_sample.hpp).
 // ---------------------------------------------------------------------------
 struct SampleArgs {
   const double* plan;
@@ -116,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void kg_theta(DevParams p, SampleArgs a) {
   if (i >= p.npix) return;
   const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
   const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
-  const uint32_t st = theta_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
+  const uint32_t st = theta_pixel<DO_PHI_T, double, false>(a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
   if (a.status && st) atomicOr(&a.status[i], st);
 }
 
@@ -126,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void kg_radial(DevParams p, SampleArgs a) {
   if (i >= p.npix) return;
   const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
   const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
-  const uint32_t st = radial_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
+  const uint32_t st = radial_pixel<DO_PHI_T, double, false>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j0, j1);
   if (a.status && st) atomicOr(&a.status[i], st);
 }
 
@@ -158,13 +167,25 @@ __device__ inline void unpack_item(unsigned long long e, int64_t& i, int64_t& j,
   walked = (e >> 63) != 0ull;
 }
 
+// grid (pixel blocks, heads): the direct evaluations of the anchor pass
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_theta_anchor(DevParams p, SampleArgs a, Records rt, int64_t nrec) {
+__global__ __launch_bounds__(kBlock) void kg_theta_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
-  for (int64_t rec = blockIdx.y; rec < nrec; rec += gridDim.y)
-    st |= theta_anchor_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, rec, rt);
+  for (int64_t hd = blockIdx.y; hd < nhead; hd += gridDim.y)
+    st |= theta_head_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, hd, rt);
+  if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+// grid (pixel blocks, anchor blocks): the long steps of the anchor pass
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_theta_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  uint32_t st = 0u;
+  for (int64_t b = blockIdx.y; b < nblk; b += gridDim.y)
+    st |= theta_far_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, b, rt);
   if (a.status && st) atomicOr(&a.status[i], st);
 }
 
@@ -194,12 +215,22 @@ __global__ __launch_bounds__(kBlock) void kg_theta_fixup(DevParams p, SampleArgs
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_radial_anchor(DevParams p, SampleArgs a, Records rt, int64_t nrec) {
+__global__ __launch_bounds__(kBlock) void kg_radial_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
-  for (int64_t rec = blockIdx.y; rec < nrec; rec += gridDim.y)
-    st |= radial_anchor_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, rec, rt);
+  for (int64_t hd = blockIdx.y; hd < nhead; hd += gridDim.y)
+    st |= radial_head_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, hd, rt);
+  if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kBlock) void kg_radial_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  uint32_t st = 0u;
+  for (int64_t b = blockIdx.y; b < nblk; b += gridDim.y)
+    st |= radial_far_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, b, rt);
   if (a.status && st) atomicOr(&a.status[i], st);
 }
 
@@ -310,7 +341,7 @@ int env_chunks() {
 
 // Workspace: the flag page, the plan [PL_N][npix] and, for a walked grid
 // (ngeo > 2), the theta and radial record tables and the fixup queue (one
-// item per 16-sample segment and pixel bounds every pass's pushes).
+// item per kFixupSpan samples and pixel bounds every pass's pushes).
 struct WsLayout {
   size_t plan, threc, rrec, queue, total;
   int64_t nth, nr;
@@ -329,7 +360,7 @@ WsLayout ws_layout(int64_t npix, int64_t ngeo) {
     off += static_cast<size_t>(L.nth) * kThFields * np * sizeof(double);
     L.rrec = off;
     off += static_cast<size_t>(L.nr) * kRFields * np * sizeof(double);
-    L.qcap = static_cast<unsigned long long>(np) * static_cast<unsigned long long>(seg_count(ngeo, kAnchorEvery));
+    L.qcap = static_cast<unsigned long long>(np) * static_cast<unsigned long long>(seg_count(ngeo, kFixupSpan));
     L.queue = off;
     off += static_cast<size_t>(L.qcap) * sizeof(unsigned long long);
   }
@@ -370,8 +401,8 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   const WsLayout L = ws_layout(d.npix, d.ngeo);
   char* base = static_cast<char*>(ws);
   uint32_t* flags = static_cast<uint32_t*>(ws);
-  const Records rth{reinterpret_cast<double*>(base + L.threc), d.npix};
-  const Records rr{reinterpret_cast<double*>(base + L.rrec), d.npix};
+  const Records rth{reinterpret_cast<double*>(base + L.threc), d.npix, 0};
+  const Records rr{reinterpret_cast<double*>(base + L.rrec), d.npix, 0};
   unsigned long long* items = reinterpret_cast<unsigned long long*>(base + L.queue);
   const FixupQueue qth{items, flags + kThetaCount, L.qcap};
   const FixupQueue qr{items, flags + kRadialCount, L.qcap};
@@ -382,14 +413,19 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   hipLaunchKernelGGL(kg_clear_counters, dim3(1), dim3(64), 0, st, flags);
   int rc = check_launch("kg_clear_counters");
   if (rc) return rc;
-  hipLaunchKernelGGL(kg_theta_anchor<DO_PHI_T>, dim3(nb, recs(L.nth)), dim3(kBlock), 0, st, d, a, rth, L.nth);
-  if ((rc = check_launch("kg_theta_anchor"))) return rc;
+  const int64_t nblk = anchor_blocks(d.ngeo), nhead = anchor_heads(d.ngeo);
+  hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kBlock), 0, st, d, a, rth, nhead);
+  if ((rc = check_launch("kg_theta_heads"))) return rc;
+  hipLaunchKernelGGL(kg_theta_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kBlock), 0, st, d, a, rth, nblk);
+  if ((rc = check_launch("kg_theta_far"))) return rc;
   hipLaunchKernelGGL(kg_theta_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rth, qth);
   if ((rc = check_launch("kg_theta_walk"))) return rc;
   hipLaunchKernelGGL(kg_theta_fixup<DO_PHI_T>, fgrid, dim3(kBlock), 0, st, d, a, qth);
   if ((rc = check_launch("kg_theta_fixup"))) return rc;
-  hipLaunchKernelGGL(kg_radial_anchor<DO_PHI_T>, dim3(nb, recs(L.nr)), dim3(kBlock), 0, st, d, a, rr, L.nr);
-  if ((rc = check_launch("kg_radial_anchor"))) return rc;
+  hipLaunchKernelGGL(kg_radial_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kBlock), 0, st, d, a, rr, nhead);
+  if ((rc = check_launch("kg_radial_heads"))) return rc;
+  hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kBlock), 0, st, d, a, rr, nblk);
+  if ((rc = check_launch("kg_radial_far"))) return rc;
   hipLaunchKernelGGL(kg_radial_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rr, qr);
   if ((rc = check_launch("kg_radial_walk"))) return rc;
   hipLaunchKernelGGL(kg_radial_fixup<DO_PHI_T>, fgrid, dim3(kBlock), 0, st, d, a, qr);
@@ -408,9 +444,9 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   const int chunks = kgeo_sample_chunks(d.npix, ngeo);
   dim3 grid(nb, static_cast<unsigned>(chunks));
   SampleArgs a{plan, gflags, SampleOut{tau_in, tau, t, r, th, ph, sig, d.npix, ngeo}, status, chunks};
-  // the linspace grid walks (three passes); caller-given Mino times and
-  // grids of at most two samples are evaluated directly in one pass
-  if (!tau_in && ngeo > 2 && std::getenv("KGEO_ONE_PASS") == nullptr)
+  // the linspace grid walks (heads, far, walk, fixup passes); caller-given
+  // Mino times and grids of at most two samples are evaluated directly
+  if (!tau_in && ngeo > 2)
     return d.do_phi_t ? run_walk<true>(d, a, ws, grid, st) : run_walk<false>(d, a, ws, grid, st);
   if (d.do_phi_t) {
     hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);

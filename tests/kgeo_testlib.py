@@ -191,6 +191,8 @@ class Host:
         outs["nmax_eq"] = nm
         outs["status"] = st
         outs["nfix"] = nfix
+        # the kernels' fixup queue holds npix * ceil(ngeo / 8) items per pass
+        assert np.all(nfix >= 0), "fixup queue over the kernels' capacity"
         return outs
 
 

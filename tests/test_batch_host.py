@@ -46,8 +46,11 @@ def test_layout_offsets_and_workspace():
                                               for im in ragged])
     slots = sum((n + 63) // 64 * 64 for n in (1, 63, 64, 65, 1000))
     plan_slots = (_lib.workspace_bytes(1000) - 256) // (1000 * 8)
+    # per pixel slot at ngeo = 10: the plan, theta records (ceil(10/32) + 3 of 6
+    # fields), radial records (ceil(10/16) + 3 of 13 fields), two queue items
+    core = slots * 8 * (plan_slots + 4 * 6 + 4 * 13 + 2)
     ws = lib.kgeo_batch_workspace_bytes(params, len(ragged))
-    assert ws >= slots * plan_slots * 8 and ws - slots * plan_slots * 8 < 4096
+    assert ws >= core and ws - core < 4096
 
 
 def test_batch_argument_errors_without_gpu():

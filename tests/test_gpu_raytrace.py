@@ -111,35 +111,43 @@ def test_full_size_invariants_and_determinism():
     torch.cuda.empty_cache()
 
 
-def test_three_passes_match_one_pass():
-    """The anchor / walk / fixup kernels against one combined kernel per
-    coordinate (KGEO_ONE_PASS) at 512x512, ngeo = 500: the same arithmetic in
-    another order, equal up to the FMA contractions each kernel's code picks
-    (theta bit-identical here).  Those last-bit differences stay near 1e-16
-    except where the value amplifies them: r far out (r = r_o at 1e4, where
-    r(sn^2) has a pole nearby: ~1e-13), the log poles of the third-kind
-    integrals and the horizon end of infalling rays (up to ~1e-10).  All
-    samples within 1e-9, 99.99 % within 1e-12."""
+def test_walk_matches_direct_evaluation():
+    """The walking passes (heads, far, walk, fixup) against direct
+    evaluation at the same Mino times (kgeo_coords_at_tau: Landen ladder and
+    Carlson duplication at every sample) at 512x512, ngeo = 500, every
+    sample: 99.99 % within 5e-12 and every sample within 4e-11, or, where the
+    value itself is ill-conditioned (log poles of the third-kind integrals, r
+    far out, the horizon end of infalling rays), within 100 times its change
+    under a 1e-14 relative change of tau plus 4e-11."""
     import torch
+    from kgeo_amd.raytrace import coords_at_tau_device
     a, th = C2["a"], np.deg2rad(C2["th_deg"])
     A, B = grid(C2["window"], C2["n"])
-    g3 = _gpu(a, C2["r_o"], th, A, B, C2["ngeo"])
-    os.environ["KGEO_ONE_PASS"] = "1"
-    try:
-        g1 = _gpu(a, C2["r_o"], th, A, B, C2["ngeo"])
-    finally:
-        del os.environ["KGEO_ONE_PASS"]
-    assert np.array_equal(g3["status"], g1["status"])
-    assert np.array_equal(g3["tau"], g1["tau"])
+    g = _gpu(a, C2["r_o"], th, A, B, C2["ngeo"])
+    s0, (t0, r0, th0, ph0) = coords_at_tau_device(a, C2["r_o"], th, A, B, g["tau"])
+    direct = dict(t=t0.cpu().numpy(), r=r0.cpu().numpy(), th=th0.cpu().numpy(), ph=ph0.cpu().numpy(),
+                  sig=s0.cpu().numpy())
+    del s0, t0, r0, th0, ph0
+    errs = {}
     for k in ("t", "r", "th", "ph", "sig"):
-        assert np.array_equal(np.isnan(g3[k]), np.isnan(g1[k])), k
-        e = np.nan_to_num(errors(g3[k], g1[k], k))
-        j, i = np.unravel_index(int(np.argmax(e)), e.shape)
+        assert np.array_equal(np.isnan(g[k]), np.isnan(direct[k])), k
+        e = np.nan_to_num(errors(g[k], direct[k], k))
         q = float(np.quantile(e, 0.9999))
-        print(f"\n[three vs one pass] {k:3s} max {e.max():.2e} (sample {j}, pixel {i}, rclass "
-              f"{int(g3['rclass'][i])})  99.99% {q:.2e}  samples > 1e-12: {int(np.sum(e > 1e-12))}")
-        assert e.max() < 1e-9 and q < 1e-12, k
-    del g3, g1
+        print(f"\n[walk vs direct] {k:3s} max {e.max():.2e}  99.99% {q:.2e}  samples > 4e-11: {int(np.sum(e > 4e-11))}")
+        assert q < 5e-12, k
+        errs[k] = e
+    # conditioning of the samples beyond 4e-11
+    cols = np.unique(np.concatenate([np.nonzero(e > 4e-11)[1] for e in errs.values()]))
+    if cols.size:
+        tau = g["tau"][:, cols]
+        s1, (t1, r1, th1, ph1) = coords_at_tau_device(a, C2["r_o"], th, A[cols], B[cols], tau * (1.0 + 1.0e-14))
+        nudge = dict(t=t1, r=r1, th=th1, ph=ph1, sig=s1)
+        for k, e in errs.items():
+            sens = np.nan_to_num(errors(nudge[k].cpu().numpy(), direct[k][:, cols], k))
+            ec = e[:, cols]
+            assert np.all((ec < 4e-11) | (ec < 100.0 * sens + 4e-11)), k
+        print(f"[walk vs direct] {cols.size} rays beyond 4e-11, all on ill-conditioned samples")
+    del g, direct
     torch.cuda.empty_cache()
 
 
