@@ -7,9 +7,10 @@ r_o = 1e4, ngeo = 500 Mino-time samples per ray: per-pixel setup (kgeo_plan)
 plus all 131,072,000 samples with r, theta, phi, t written to HBM
 (kgeo_sample).  Inputs are resident in HBM before the timed region.
 
-Prints ONE JSON line with the metric, the roofline of the sample kernels
-(kg_theta + kg_radial, FP64 vector ALU; timed with HIP events on the launch
-stream; algorithmic FLOPs op-counted on the kernels' own loops) and the CPU
+Prints ONE JSON line with the metric, the roofline of the sample passes
+(the heads, far, walk and fixup kernels of theta and r, FP64 vector ALU;
+timed with HIP events on the launch stream; algorithmic FLOPs op-counted on
+the kernels' own code) and the CPU
 baseline (the oracle, i.e. the reference algorithm in NumPy/SciPy, on a
 bounded seeded sample of the same workload on this host).
 
@@ -128,12 +129,12 @@ def cpu_baseline(A, B, th, px_per_core=768, seed=1):
             "scipy": scipy.__version__}
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "kg_theta_kg_radial_512x512_ngeo500_pmc.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "kg_sample_passes_512x512_ngeo500_pmc.txt")
 
 
 def read_profile_value(key):
-    """A figure from the committed rocprofv3 counter summary of kg_theta +
-    kg_radial at this workload (FETCH_SIZE + WRITE_SIZE scaled by a plain
+    """A figure from the committed rocprofv3 counter summary of the sample
+    passes at this workload (FETCH_SIZE + WRITE_SIZE scaled by a plain
     copy's reading; hardware FP64 FLOP count), or None."""
     if not os.path.exists(PMC_SUMMARY):
         return None
@@ -388,7 +389,7 @@ def main():
     achieved_tf = flops_s * samples / (samp_ms * 1e-3) / 1e12
     bytes_per_launch = 32.0 * samples
     hbm_gbs = bytes_per_launch / (samp_ms * 1e-3) / 1e9
-    roofline = {"bound": "valu-fp64", "kernel": "kg_theta + kg_radial", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+    roofline = {"bound": "valu-fp64", "kernel": "sample passes (kg_theta_* + kg_radial_*: heads, far, walk, fixup)", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS,
                 "traffic": read_profile_value("traffic_bytes_per_launch"),
                 "flops_per_sample": flops_s, "algorithmic_bytes_per_launch": bytes_per_launch,

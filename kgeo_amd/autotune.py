@@ -1,13 +1,13 @@
 """Autotuner for the batched survey's launch shape.
 
 Two knobs (kgeo_batch_tuning in include/kgeo.h):
-  samples_per_item  Mino-time samples per work item; a work item is one
-                    64-pixel tile x this many samples, so it sets how many
-                    lanes share one pixel's ngeo range (ceil(ngeo / this)):
-                    small images need several to fill 256 CUs, large ones
-                    want long runs per lane to amortise the per-pixel plan
-                    load and Jacobi ladder set-up.
-  waves_per_cu      size of the persistent grid (CUs x this waves).
+  samples_per_item  Mino-time samples per walk work item; a walk item is
+                    one 64-pixel tile x this many samples, so it sets how
+                    many lanes share one pixel's ngeo range
+                    (ceil(ngeo / this)): small images need several to fill
+                    256 CUs, large ones want long runs per lane to amortise
+                    the per-lane set-up (plan slots, step record).
+  waves_per_cu      size of each pass's persistent grid (CUs x this waves).
 
 Choices are cached in a JSON file keyed by (gfx arch, log2 pixel-count
 bucket, log2 ngeo bucket) of the survey.  The cache lives at

@@ -1,12 +1,14 @@
-"""Batched survey: many ``raytrace_ana`` images in one persistent launch.
+"""Batched survey: many ``raytrace_ana`` images in one graph launch.
 
 The reference traces one image per ``raytrace_ana`` call
 (kerr_raytracing_ana.py:46-122); a survey over spins, inclinations,
 resolutions and ngeo is a Python loop of such calls.  ``BatchSurvey`` runs
 the whole survey as one HIP graph: a setup kernel over every pixel of every
-image, then one persistent kernel whose waves pull (image, 64-pixel tile,
-Mino-time chunk) work items from an atomic queue (kgeo_batch.hip).  Each
-image's results equal a plain ``raytrace_ana_device`` call on that image.
+image, then for theta and r the plain path's passes (direct heads, long
+steps, walk, fixups) as persistent kernels whose waves pull (image, 64-pixel
+tile, unit) work items from one atomic queue per pass (kgeo_batch.hip).
+Each image's results equal a plain ``raytrace_ana_device`` call on that
+image to the last bits.
 
 Outputs live in one arena per coordinate (t, r, th, ph; tau and sigma are
 not stored): image k is the (ngeo_k, npix_k) block at ``out_off[k]``.

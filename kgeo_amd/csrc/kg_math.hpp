@@ -60,6 +60,16 @@ KG_INLINE double kfloor(double x) { return ::floor(x); }
 KG_INLINE double krint(double x) { return ::rint(x); }
 KG_INLINE bool kfinite(double x) { return kfabs(x) < INFINITY; }  // false for NaN
 
+// True on every active lane of the wave when p holds on any of them (the
+// host build evaluates one lane at a time: p itself).
+KG_INLINE bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __any(p ? 1 : 0) != 0;
+#else
+  return p;
+#endif
+}
+
 // Reciprocal to within an ulp or two, for the walk's well-scaled values:
 // the hardware approximation refined by two Newton steps on the GPU (five
 // instructions against the ten of a correctly rounded division), a plain

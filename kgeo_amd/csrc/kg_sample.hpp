@@ -60,6 +60,24 @@ KG_INLINE bool theta_store(const SampleOut& o, int64_t idx, double tau, const T&
   return !kfinite(val(th));
 }
 
+// Gph, Gt: the theta pass's G_phi, G_t of this sample (parked in ph, t), when
+// the caller has loaded them ahead of the arithmetic (radial_store_g).
+template <class T, bool DO_PHI_T>
+KG_INLINE bool radial_store_g(const SampleOut& o, const CombineConsts<T>& cmb, int64_t idx, double tau,
+                              const RadialOut<T>& ro, const T& Gph, const T& Gt) {
+  if (o.r) o.r[idx] = val(ro.r);
+  bool bad = !kfinite(val(ro.r));
+  if (DO_PHI_T) {
+    Sample<T> s;
+    combine(cmb, T(tau), ro, T(0.0), Gph, Gt, s);
+    o.t[idx] = val(s.t);
+    o.ph[idx] = val(s.ph);
+    if (o.sig) o.sig[idx] = val(s.sig);
+    bad |= !(kfinite(val(s.t)) && kfinite(val(s.ph)) && kfinite(val(s.sig)));
+  }
+  return bad;
+}
+
 template <class T, bool DO_PHI_T>
 KG_INLINE bool radial_store(const SampleOut& o, const CombineConsts<T>& cmb, int64_t idx, double tau,
                             const RadialOut<T>& ro) {
@@ -174,10 +192,11 @@ template <class T> KG_INLINE T block_log_enter(const T& lg) {
 
 // One walked step of the case 3 / 4 block logs from state a to state b;
 // false (lb then unspecified) where a log leaves [-kLogGuard, kLogGuard].
-template <class T, int CASE>
+// ADAPT as in walk_step.
+template <class T, int CASE, bool ADAPT = false>
 KG_INLINE bool radial_block_step(const RadialConsts<T>& rc, const WalkState<T, 3>& a, const WalkState<T, 3>& b,
                                  T (&lb)[3]) {
-  T a0[3], b0[3], a1[3], b1[3], den[3], iden[3];
+  T a0[3], b0[3], a1[3], b1[3], den[3], iden[3], y[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     block_log_args<T, CASE>(rc, q, a.s, a.d, a0[q], b0[q]);
@@ -185,10 +204,18 @@ KG_INLINE bool radial_block_step(const RadialConsts<T>& rc, const WalkState<T, 3
     den[q] = walk_log_den(a0[q], b0[q], a1[q], b1[q]);
   }
   shared_reciprocals<T, 3>(den, iden);
+  double ymax = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    y[q] = walk_log_y(a0[q], b0[q], a1[q], b1[q], iden[q]);
+    const double y2 = kfabs(val(y[q] * y[q]));
+    if (y2 <= kStepZ2) ymax = kmax(ymax, y2);
+  }
+  const int nt = rc_terms<ADAPT>(ymax);
   bool ok = true;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    lb[q] = walk_log(lb[q], a0[q], b0[q], a1[q], b1[q], iden[q]);
+    lb[q] = walk_log(lb[q], y[q], a1[q], b1[q], nt);
     ok = ok && kfabs(val(lb[q])) <= kLogGuard;
   }
   return ok;
@@ -594,7 +621,7 @@ KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t 
     }
     if (queued) continue;
     WalkState<T, 1> nx;
-    if (!walk_step<T, 1, DO_PHI_T>(st, w, nx)) {
+    if (!walk_step<T, 1, DO_PHI_T, false, true>(st, w, nx)) {
       push_range(push, i, j, j - j % L + L, last);
       queued = true;
       continue;
@@ -761,8 +788,14 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
       continue;
     }
     if (queued) continue;
+    // the theta pass's G_phi, G_t of this sample, loaded before the step so
+    // that their memory latency overlaps its arithmetic
+    const int64_t idx = j * o.npix + i;
+    const T gph = DO_PHI_T ? T(o.ph[idx]) : T(0.0);
+    const T gt = DO_PHI_T ? T(o.t[idx]) : T(0.0);
     WalkState<T, 3> nx;
-    if (!walk_step<T, 3, DO_PHI_T>(st, w, nx) || (kBlocks && !radial_block_step<T, CASE>(rc, st, nx, lb))) {
+    if (!walk_step<T, 3, DO_PHI_T, false, true>(st, w, nx) ||
+        (kBlocks && !radial_block_step<T, CASE, true>(rc, st, nx, lb))) {
       push_range(push, i, j, j - j % L + L, last);
       queued = true;
       continue;
@@ -771,7 +804,8 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
     const double tau = mino_tau(j, o.ngeo, stop, false);
     RState<T> rs;
     radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
-    bad |= radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, j, tau, rs);
+    RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
+    bad |= radial_store_g<T, DO_PHI_T>(o, cmb, idx, tau, ro, gph, gt);
   }
   return bad;
 }

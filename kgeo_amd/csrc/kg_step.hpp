@@ -43,24 +43,28 @@ constexpr double kLogGuard = 10.0;
 constexpr int kAnchorEvery = 16;       // radial cases 3 and 4
 constexpr int kAnchorEveryLong = 32;   // theta and radial case 2
 
-// R_C(1, 1 - z2) = sum_k z2^k / (2k + 1) for |z2| <= kStepZ2
-// (0.04^15 / 31 < 1e-22: the truncation is below rounding)
-template <class T> KG_INLINE T rc_one_series(const T& z2) {
-  T g = T(1.0 / 29.0);
-  g = g * z2 + 1.0 / 27.0;
-  g = g * z2 + 1.0 / 25.0;
-  g = g * z2 + 1.0 / 23.0;
-  g = g * z2 + 1.0 / 21.0;
-  g = g * z2 + 1.0 / 19.0;
-  g = g * z2 + 1.0 / 17.0;
-  g = g * z2 + 1.0 / 15.0;
-  g = g * z2 + 1.0 / 13.0;
-  g = g * z2 + 1.0 / 11.0;
-  g = g * z2 + 1.0 / 9.0;
-  g = g * z2 + 1.0 / 7.0;
-  g = g * z2 + 1.0 / 5.0;
-  g = g * z2 + 1.0 / 3.0;
-  return g * z2 + 1.0;
+// R_C(1, 1 - z2) = sum_k z2^k / (2k + 1), the first NT terms.
+template <int NT, class T> KG_INLINE T rc_one_terms(const T& z2) {
+  T g = T(1.0 / (2 * NT - 1));
+#pragma unroll
+  for (int k = NT - 2; k >= 0; --k) g = g * z2 + 1.0 / (2 * k + 1);
+  return g;
+}
+
+// Terms the series needs for |z2| <= amax, to below rounding:
+// 12 for kStepZ2 (0.04^12 / 25 < 1e-18), 6 for 2e-3, 4 for 1e-4
+// (1e-16 / 9).  ADAPT: one count for the whole wave (a scalar branch; the
+// walk kernels, where a wave's lanes are 64 neighbouring pixels at one
+// sample, so the choice does not depend on the launch shape); otherwise 12.
+template <bool ADAPT> KG_INLINE int rc_terms(double amax) {
+  if (!ADAPT || wave_any(!(amax <= 2.0e-3))) return 12;
+  return wave_any(amax > 1.0e-4) ? 6 : 4;
+}
+
+template <class T> KG_INLINE T rc_one_series(const T& z2, int nt = 12) {
+  if (nt == 4) return rc_one_terms<4>(z2);
+  if (nt == 6) return rc_one_terms<6>(z2);
+  return rc_one_terms<12>(z2);
 }
 
 // Jacobi state (sn, cn, dn), (E - F)/m and NP third-kind integrals at one
@@ -169,12 +173,16 @@ template <class T, int NP> KG_INLINE void shared_reciprocals(const T (&x)[NP], T
 // One step of a walked log ratio Lg = ln|(b + a)/(b - a)| from (a0, b0) to
 // (a1, b1): Lg1 = Lg0 + ln|(1 + y)/(1 - y)|, y = (a1 b0 - a0 b1)/(b0 b1 - a0 a1),
 // by the series while |y^2| <= kStepZ2, else from its closed form.  The
-// caller passes 1/(b0 b1 - a0 a1) (walk_log_den), shared across integrals.
+// caller passes 1/(b0 b1 - a0 a1) (walk_log_den), shared across integrals,
+// and the series' term count.
+template <class T> KG_INLINE T walk_log_y(const T& a0, const T& b0, const T& a1, const T& b1, const T& inv_den) {
+  return (a1 * b0 - a0 * b1) * inv_den;
+}
+
 template <class T>
-KG_INLINE T walk_log(const T& Lg0, const T& a0, const T& b0, const T& a1, const T& b1, const T& inv_den) {
-  const T y = (a1 * b0 - a0 * b1) * inv_den;
+KG_INLINE T walk_log(const T& Lg0, const T& y, const T& a1, const T& b1, int nt) {
   const T y2 = y * y;
-  if (kfabs(val(y2)) <= kStepZ2) return Lg0 + 2.0 * y * rc_one_series(y2);
+  if (kfabs(val(y2)) <= kStepZ2) return Lg0 + 2.0 * y * rc_one_series(y2, nt);
   return log_ratio(a1, b1);
 }
 
@@ -197,7 +205,7 @@ constexpr double kFarZ2 = 0.5;
 // One step a -> a + du.  Returns false (leaving `out` unspecified) when the
 // step is not well conditioned; the caller then evaluates directly.  FAR
 // admits z^2 < kFarZ2 (closed-form R_C) instead of |z^2| <= kStepZ2.
-template <class T, int NP, bool INTEGRALS, bool FAR = false>
+template <class T, int NP, bool INTEGRALS, bool FAR = false, bool ADAPT = false>
 KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out) {
   const WalkState<T, NP>& b = w.b;
   const T ss = a.s * b.s;
@@ -222,6 +230,14 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
   }
   T ipv[NP];
   shared_reciprocals<T, NP>(pv, ipv);
+  T z2v[NP];
+  double zmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    z2v[i] = w.w2[i] * (q * ipv[i]) * (q * ipv[i]);
+    if (kfabs(val(z2v[i])) <= kStepZ2) zmax = kmax(zmax, kfabs(val(z2v[i])));  // others are refused
+  }
+  const int nt = rc_terms<ADAPT>(zmax);
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     // 1 - n sn^2 at both ends and p must share a sign: no pole of the
@@ -231,7 +247,7 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
     const T& p = pv[i];
     const T& ip = ipv[i];
     const T nq = w.n[i] * q * ip;
-    const T z2 = w.w2[i] * (q * ip) * (q * ip);
+    const T& z2 = z2v[i];
     const bool series = kfabs(val(z2)) <= kStepZ2;
     const bool psign = (val(p) > 0.0) == (val(r1) > 0.0);
     // next to the integrand's pole (1 - n sn^2 -> 0: n > 1, or n -> 1 at
@@ -243,7 +259,7 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
          (circ || (psign && (series || (FAR && val(z2) <= kFarZ2))));
     T corr;
     if (circ) corr = w.ncw[i] * katan2(w.cw[i] * q, p);
-    else corr = nq * ((!FAR || series) ? rc_one_series(z2) : rc_1pe(-z2));
+    else corr = nq * ((!FAR || series) ? rc_one_series(z2, nt) : rc_1pe(-z2));
     out.P[i] = a.P[i] + b.P[i] + corr;
     ok = ok && kfinite(val(out.P[i]));
   }
@@ -254,13 +270,21 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
 #pragma unroll
   for (int i = 0; i < NP; ++i) yd[i] = w.tr[i] ? cdn - (w.pt[i] * w.pt[i]) * (S * a.s) : T(1.0);
   shared_reciprocals<T, NP>(yd, iyd);
+  T yv[NP];
+  double ymax = 0.0;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    yv[i] = w.pt[i] * num * iyd[i];
+    if (w.tr[i] && kfabs(val(yv[i] * yv[i])) <= kStepZ2) ymax = kmax(ymax, kfabs(val(yv[i] * yv[i])));
+  }
+  const int nty = rc_terms<ADAPT>(ymax);
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     if (w.tr[i]) {
-      const T y = w.pt[i] * num * iyd[i];
+      const T& y = yv[i];
       const T y2 = y * y;
       if (kfabs(val(y2)) <= kStepZ2) {
-        out.Lg[i] = a.Lg[i] + 2.0 * y * rc_one_series(y2);
+        out.Lg[i] = a.Lg[i] + 2.0 * y * rc_one_series(y2, nty);
       } else {
         out.Lg[i] = log_ratio(w.pt[i] * S, C * Dn);
       }

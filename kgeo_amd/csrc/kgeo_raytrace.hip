@@ -52,6 +52,11 @@ int check_launch(const char* what) {
 }
 
 constexpr int kBlock = 256;
+// The sample passes run one wave per workgroup: their waves differ widely in
+// cost (radial case, refused steps), and a CU's slot is freed per workgroup,
+// so a multi-wave workgroup would hold its SIMDs until its slowest wave ends.
+// (The theta walk, whose waves cost alike, keeps kBlock.)
+constexpr int kPassBlock = 64;
 constexpr size_t kFlagBytes = 256;
 
 struct DevParams {
@@ -169,8 +174,8 @@ __device__ inline void unpack_item(unsigned long long e, int64_t& i, int64_t& j,
 
 // grid (pixel blocks, heads): the direct evaluations of the anchor pass
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_theta_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPassBlock) void kg_theta_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
   for (int64_t hd = blockIdx.y; hd < nhead; hd += gridDim.y)
@@ -180,8 +185,8 @@ __global__ __launch_bounds__(kBlock) void kg_theta_heads(DevParams p, SampleArgs
 
 // grid (pixel blocks, anchor blocks): the long steps of the anchor pass
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_theta_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPassBlock) void kg_theta_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
   for (int64_t b = blockIdx.y; b < nblk; b += gridDim.y)
@@ -201,11 +206,11 @@ __global__ __launch_bounds__(kBlock) void kg_theta_walk(DevParams p, SampleArgs 
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_theta_fixup(DevParams p, SampleArgs a, FixupQueue q) {
+__global__ __launch_bounds__(kPassBlock) void kg_theta_fixup(DevParams p, SampleArgs a, FixupQueue q) {
   const unsigned long long c = *q.count;
   const unsigned long long n = c < q.cap ? c : q.cap;
-  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
-       e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kPassBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kPassBlock) {
     int64_t i, j;
     bool walked;
     unpack_item(q.items[e], i, j, walked);
@@ -215,8 +220,8 @@ __global__ __launch_bounds__(kBlock) void kg_theta_fixup(DevParams p, SampleArgs
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_radial_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPassBlock) void kg_radial_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
   for (int64_t hd = blockIdx.y; hd < nhead; hd += gridDim.y)
@@ -225,8 +230,8 @@ __global__ __launch_bounds__(kBlock) void kg_radial_heads(DevParams p, SampleArg
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_radial_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPassBlock) void kg_radial_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
   for (int64_t b = blockIdx.y; b < nblk; b += gridDim.y)
@@ -235,8 +240,8 @@ __global__ __launch_bounds__(kBlock) void kg_radial_far(DevParams p, SampleArgs 
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void kg_radial_walk(DevParams p, SampleArgs a, Records rt, FixupQueue q) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(2))) void kg_radial_walk(DevParams p, SampleArgs a, Records rt, FixupQueue q) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
   const int64_t j0 = chunk_start(p.ngeo, blockIdx.y, a.chunks);
   const int64_t j1 = chunk_start(p.ngeo, blockIdx.y + 1, a.chunks);
@@ -246,11 +251,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kBlock) void kg_radial_fixup(DevParams p, SampleArgs a, FixupQueue q) {
+__global__ __launch_bounds__(kPassBlock) void kg_radial_fixup(DevParams p, SampleArgs a, FixupQueue q) {
   const unsigned long long c = *q.count;
   const unsigned long long n = c < q.cap ? c : q.cap;
-  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
-       e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kPassBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kPassBlock) {
     int64_t i, j;
     bool walked;
     unpack_item(q.items[e], i, j, walked);
@@ -319,13 +324,14 @@ __global__ __launch_bounds__(kBlock) void kg_counts(DevParams p, const double* _
   }
 }
 
-// Samples per lane chunking: split ngeo so that about 2^20 (pixel, chunk)
-// lanes exist (256 CUs x 64 waves of 64 over the kernel's lifetime; more
+// Samples per lane chunking: split ngeo so that about 2^21 (pixel, chunk)
+// lanes exist (256 CUs x 128 waves of 64 over the kernels' lifetime: more
 // lanes than the chip holds at once lets waves that finish early be
-// replaced), keeping chunks >= 32 samples so the walk's per-chunk set-up
-// (two direct evaluations) stays a few percent of the work.
+// replaced; measured at 512^2 x 500, 8 chunks run 2 % faster than 4 and as
+// fast as 15), keeping chunks >= 32 samples so the walk's per-chunk set-up
+// stays a few percent of the work.
 int pick_chunks(int64_t npix, int64_t ngeo) {
-  const int64_t target_lanes = 256LL * 64 * 64;
+  const int64_t target_lanes = 256LL * 128 * 64;
   int64_t c = (target_lanes + npix - 1) / (npix > 0 ? npix : 1);
   if (c < 1) c = 1;
   const int64_t max_c = ngeo / 32 > 0 ? ngeo / 32 : 1;
@@ -406,29 +412,30 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   unsigned long long* items = reinterpret_cast<unsigned long long*>(base + L.queue);
   const FixupQueue qth{items, flags + kThetaCount, L.qcap};
   const FixupQueue qr{items, flags + kRadialCount, L.qcap};
-  const unsigned nb = grid.x;
-  const unsigned long long fb = (L.qcap + kBlock - 1) / kBlock;
-  const dim3 fgrid(static_cast<unsigned>(fb < 1024ull ? fb : 1024ull));
+  const unsigned nb = static_cast<unsigned>((d.npix + kPassBlock - 1) / kPassBlock);
+  const dim3 wgrid(nb, grid.y);
+  const unsigned long long fb = (L.qcap + kPassBlock - 1) / kPassBlock;
+  const dim3 fgrid(static_cast<unsigned>(fb < 4096ull ? fb : 4096ull));
   auto recs = [](int64_t n) { return static_cast<unsigned>(n < 65535 ? n : 65535); };
   hipLaunchKernelGGL(kg_clear_counters, dim3(1), dim3(64), 0, st, flags);
   int rc = check_launch("kg_clear_counters");
   if (rc) return rc;
   const int64_t nblk = anchor_blocks(d.ngeo), nhead = anchor_heads(d.ngeo);
-  hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kBlock), 0, st, d, a, rth, nhead);
+  hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rth, nhead);
   if ((rc = check_launch("kg_theta_heads"))) return rc;
-  hipLaunchKernelGGL(kg_theta_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kBlock), 0, st, d, a, rth, nblk);
+  hipLaunchKernelGGL(kg_theta_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rth, nblk);
   if ((rc = check_launch("kg_theta_far"))) return rc;
   hipLaunchKernelGGL(kg_theta_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rth, qth);
   if ((rc = check_launch("kg_theta_walk"))) return rc;
-  hipLaunchKernelGGL(kg_theta_fixup<DO_PHI_T>, fgrid, dim3(kBlock), 0, st, d, a, qth);
+  hipLaunchKernelGGL(kg_theta_fixup<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, qth);
   if ((rc = check_launch("kg_theta_fixup"))) return rc;
-  hipLaunchKernelGGL(kg_radial_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kBlock), 0, st, d, a, rr, nhead);
+  hipLaunchKernelGGL(kg_radial_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rr, nhead);
   if ((rc = check_launch("kg_radial_heads"))) return rc;
-  hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kBlock), 0, st, d, a, rr, nblk);
+  hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rr, nblk);
   if ((rc = check_launch("kg_radial_far"))) return rc;
-  hipLaunchKernelGGL(kg_radial_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rr, qr);
+  hipLaunchKernelGGL(kg_radial_walk<DO_PHI_T>, wgrid, dim3(kPassBlock), 0, st, d, a, rr, qr);
   if ((rc = check_launch("kg_radial_walk"))) return rc;
-  hipLaunchKernelGGL(kg_radial_fixup<DO_PHI_T>, fgrid, dim3(kBlock), 0, st, d, a, qr);
+  hipLaunchKernelGGL(kg_radial_fixup<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, qr);
   return check_launch("kg_radial_fixup");
 }
 
