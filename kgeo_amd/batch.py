@@ -61,11 +61,16 @@ def survey_size(images):
     return pix, smp
 
 
+def _phi_t(im, default):
+    """An image's do_phi_and_t: its own "do_phi_and_t" entry, else the survey's."""
+    return bool(im.get("do_phi_and_t", default))
+
+
 def _params_array(images, r_o, maxtaufrac, do_phi_and_t):
     arr = (_lib.RayParams * len(images))()
     for k, im in enumerate(images):
         arr[k] = _lib.make_params(im["a"], r_o, im["th_o"], len(im["alpha"]), int(im["ngeo"]), maxtaufrac,
-                                  do_phi_and_t)
+                                  _phi_t(im, do_phi_and_t))
     return arr
 
 
@@ -85,7 +90,9 @@ class BatchSurvey:
     """A planned survey: device arenas, workspace and the captured graph.
 
     images: sequence of dicts with a, th_o, ngeo, alpha, beta (arrays or
-    tensors).  tuning: (samples_per_item, waves_per_cu), or None to use the
+    tensors), and optionally do_phi_and_t, which overrides the survey's
+    for that image (t and ph of an image without it are left unwritten).
+    tuning: (samples_per_item, waves_per_cu), or None to use the
     autotuner's cached choice for this survey's shape (default (64, 8) when
     nothing is cached).  ``arena`` may pass preallocated flat float64 tensors
     for t, r, th, ph (at least out_off[-1] elements each), e.g. to reuse one
@@ -111,8 +118,9 @@ class BatchSurvey:
         self.beta = torch.cat([_as_device_f64(im["beta"], dev).reshape(-1) for im in self.images])
         arena = dict(arena or {})
         self.arena = {}
+        any_phi_t = any(_phi_t(im, do_phi_and_t) for im in self.images)
         for k in KEYS:
-            if not do_phi_and_t and k in ("t", "ph"):
+            if not any_phi_t and k in ("t", "ph"):
                 self.arena[k] = None
                 continue
             a = arena.get(k)
@@ -219,12 +227,15 @@ def run_sequential(images, arena, out_off, r_o=1.0e4, maxtaufrac=MAXTAUFRAC, do_
     pix0 = 0
     for k, im in enumerate(images):
         P, G = len(im["alpha"]), int(im["ngeo"])
-        p = _lib.make_params(im["a"], r_o, im["th_o"], P, G, maxtaufrac, do_phi_and_t)
+        phi_t = _phi_t(im, do_phi_and_t)
+        p = _lib.make_params(im["a"], r_o, im["th_o"], P, G, maxtaufrac, phi_t)
         o0 = int(out_off[k])
 
         def view(key):
             a = arena.get(key)
-            return None if a is None else a[o0:o0 + P * G]
+            if a is None or (not phi_t and key in ("t", "ph")):
+                return None
+            return a[o0:o0 + P * G]
 
         def lab(key):
             a = labels.get(key)
