@@ -43,12 +43,25 @@ constexpr double kLogGuard = 10.0;
 constexpr int kAnchorEvery = 16;       // radial cases 3 and 4
 constexpr int kAnchorEveryLong = 32;   // theta and radial case 2
 
-// R_C(1, 1 - z2) = sum_k z2^k / (2k + 1), the first NT terms.
+// R_C(1, 1 - z2) = sum_k z2^k / (2k + 1), the first NT terms (NT even), by
+// Estrin's scheme: pairs 1/(4k+1) + z2/(4k+3), then combined with z2^2,
+// z2^4, ...  Its dependency depth is 1 + log2(NT) multiply-adds instead of
+// Horner's NT - 1, which matters in the walk loops, where the series sits on
+// each step's critical path.
 template <int NT, class T> KG_INLINE T rc_one_terms(const T& z2) {
-  T g = T(1.0 / (2 * NT - 1));
+  static_assert(NT % 2 == 0, "even term count");
+  T p[NT / 2];
 #pragma unroll
-  for (int k = NT - 2; k >= 0; --k) g = g * z2 + 1.0 / (2 * k + 1);
-  return g;
+  for (int k = 0; k < NT / 2; ++k) p[k] = 1.0 / (4 * k + 1) + z2 * (1.0 / (4 * k + 3));
+  T x = z2 * z2;
+#pragma unroll
+  for (int n = NT / 2; n > 1; n = (n + 1) / 2) {
+#pragma unroll
+    for (int k = 0; k < n / 2; ++k) p[k] = p[2 * k] + x * p[2 * k + 1];
+    if (n % 2) p[n / 2] = p[n - 1];
+    if (n > 2) x = x * x;
+  }
+  return p[0];
 }
 
 // Terms the series needs for |z2| <= amax, to below rounding:
@@ -202,11 +215,10 @@ template <class T> KG_INLINE T walk_log_den(const T& a0, const T& b0, const T& a
 // sign condition on p.
 constexpr double kFarZ2 = 0.5;
 
-// One step a -> a + du.  Returns false (leaving `out` unspecified) when the
-// step is not well conditioned; the caller then evaluates directly.  FAR
-// admits z^2 < kFarZ2 (closed-form R_C) instead of |z^2| <= kStepZ2.
-template <class T, int NP, bool INTEGRALS, bool FAR = false, bool ADAPT = false>
-KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out) {
+// The Jacobi part of a step a -> a + du (DLMF 22.8.1-3): sn, cn, dn at
+// a + du into out; false where one is not finite.
+template <class T, int NP>
+KG_INLINE bool walk_jacobi(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out) {
   const WalkState<T, NP>& b = w.b;
   const T ss = a.s * b.s;
   const T inv = frcp(1.0 - w.m * ss * ss);
@@ -216,8 +228,19 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
   out.s = S;
   out.c = C;
   out.d = Dn;
-  bool ok = kfinite(val(S)) && kfinite(val(C)) && kfinite(val(Dn));
-  if (!INTEGRALS) return ok;
+  return kfinite(val(S)) && kfinite(val(C)) && kfinite(val(Dn));
+}
+
+// The rest of the step a -> a + du, given out.s, out.c, out.d from
+// walk_jacobi (ok: its result): (E - F)/m, the NP third-kind integrals and
+// their log terms.
+template <class T, int NP, bool FAR = false, bool ADAPT = false>
+KG_INLINE bool walk_integrals(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out, bool ok) {
+  const WalkState<T, NP>& b = w.b;
+  const T ss = a.s * b.s;
+  const T S = out.s;
+  const T C = out.c;
+  const T Dn = out.d;
   const T q = ss * S;
   out.D = a.D + b.D - q;
   const T cd = ss * C * Dn;
@@ -294,6 +317,16 @@ KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, Wa
     }
   }
   return ok && kfinite(val(out.D));
+}
+
+// One step a -> a + du.  Returns false (leaving `out` unspecified) when the
+// step is not well conditioned; the caller then evaluates directly.  FAR
+// admits z^2 < kFarZ2 (closed-form R_C) instead of |z^2| <= kStepZ2.
+template <class T, int NP, bool INTEGRALS, bool FAR = false, bool ADAPT = false>
+KG_INLINE bool walk_step(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, WalkState<T, NP>& out) {
+  const bool ok = walk_jacobi(a, w, out);
+  if (!INTEGRALS) return ok;
+  return walk_integrals<T, NP, FAR, ADAPT>(a, w, out, ok);
 }
 
 }  // namespace kg
