@@ -1,6 +1,7 @@
 // The head / far / walk / fixup passes of the sample kernels (kg_sample.hpp,
 // kgeo_raytrace.hip) run on the CPU over all pixels, in the kernels' order:
-// theta heads, long steps, walk, fixups, then the same for r.  Used by the
+// theta heads, long steps, walk, fixups, then the same for r, whose long
+// steps are followed by the repair of blocks with a lost long step.  Used by the
 // CPU test build (T = double) and the FLOP counter (T = its counting type).
 
 #pragma once
@@ -19,6 +20,13 @@ struct HostPush {
   void operator()(int64_t i, int64_t j, bool walked) const {
     items->push_back((walked ? (1ull << 63) : 0ull) | (static_cast<uint64_t>(j) << 32) | static_cast<uint32_t>(i));
   }
+};
+
+// The far pass's repair items, on the host (the same item format).
+struct HostRepair {
+  static constexpr bool kOn = true;
+  std::vector<uint64_t>* items;
+  void operator()(int64_t i, int64_t j, bool walked) const { HostPush{items}(i, j, walked); }
 };
 
 inline void unpack_host_item(uint64_t e, int64_t& i, int64_t& j, bool& walked) {
@@ -55,14 +63,25 @@ void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, c
     int64_t i, j;
     bool w;
     unpack_host_item(e, i, j, w);
-    st[i] |= theta_fixup_pixel<DO_PHI_T, T>(plan, npix, i, gflags, o, i, j, w);
+    for (int64_t sg = 0; sg < kFixupSpan; ++sg)
+      st[i] |= theta_fixup_sample<DO_PHI_T, T>(plan, npix, i, gflags, o, i, j, w, sg);
   }
   items.clear();
   for (int64_t i = 0; i < npix; ++i)
     for (int64_t hd = 0; hd < nhead; ++hd)
       st[i] |= radial_head_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, hd, rr);
+  std::vector<uint64_t> repairs;
+  const HostRepair repair{&repairs};
   for (int64_t i = 0; i < npix; ++i)
-    for (int64_t b = 0; b < nblk; ++b) st[i] |= radial_far_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, b, rr);
+    for (int64_t b = 0; b < nblk; ++b)
+      st[i] |= radial_far_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, b, rr, repair);
+  for (uint64_t e : repairs) {
+    int64_t i, j;
+    bool w;
+    unpack_host_item(e, i, j, w);
+    for (int64_t sg = 0; sg < kAnchorBlock / kAnchorEvery; ++sg)
+      st[i] |= radial_repair_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, j, sg, rr);
+  }
   for (int64_t i = 0; i < npix; ++i)
     for (int ch = 0; ch < chunks; ++ch)
       st[i] |= radial_walk_pixel<DO_PHI_T, HostPush, T>(c, plan, npix, i, gflags, o, i, chunk_start(G, ch, chunks),
@@ -72,7 +91,8 @@ void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, c
     int64_t i, j;
     bool w;
     unpack_host_item(e, i, j, w);
-    st[i] |= radial_fixup_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, j, w);
+    for (int64_t sg = 0; sg < kFixupSpan; ++sg)
+      st[i] |= radial_fixup_sample<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, j, w, sg);
   }
 }
 

@@ -726,10 +726,21 @@ KG_INLINE bool radial_head(const SampleOut& o, const RadialConsts<T>& rc, const 
   return radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, j, tau, rs);
 }
 
+// A far pass without a repair pass: a lost long step invalidates the
+// block's remaining segment records, and the walk pass queues those segments
+// whole for the fixup pass.
+struct NoRepair {
+  static constexpr bool kOn = false;
+  KG_INLINE void operator()(int64_t, int64_t, bool) const {}
+};
+
 // Radial segment starts of block blk after its first sample (as theta_far).
-template <class T, int CASE, bool DO_PHI_T>
+// With a repair pass (Repair::kOn), the first lost long step hands segment
+// js and the rest of the block to it (repair(i, js, false)) instead.
+template <class T, int CASE, bool DO_PHI_T, class Repair = NoRepair>
 KG_INLINE bool radial_far(const SampleOut& o, const RadialConsts<T>& rc, const CombineConsts<T>& cmb, int64_t i,
-                          int64_t blk, double stop, const Records& rt, int64_t col) {
+                          int64_t blk, double stop, const Records& rt, int64_t col,
+                          const Repair& repair = Repair()) {
   constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
   constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
   const T m = rc.v[R_k];
@@ -750,6 +761,10 @@ KG_INLINE bool radial_far(const SampleOut& o, const RadialConsts<T>& rc, const C
     lost = lost || !walk_step<T, 3, DO_PHI_T, true>(st, w, nx) ||
            (kBlocks && !radial_block_step<T, CASE>(rc, st, nx, lb));
     if (lost) {
+      if (Repair::kOn) {
+        repair(i, js, false);
+        return bad;
+      }
       record_invalidate(rt, segment_record(js, kAnchorEvery), kRFields, col);
       continue;
     }
@@ -761,6 +776,36 @@ KG_INLINE bool radial_far(const SampleOut& o, const RadialConsts<T>& rc, const C
     bad |= radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, js, tau, rs);
   }
   return bad;
+}
+
+// The repair pass: segment start js of a block whose far pass lost a long
+// step at or before js, by direct evaluation; writes its record and start
+// sample, as the far pass would have.
+template <class T, int CASE, bool DO_PHI_T>
+KG_INLINE bool radial_repair(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
+                             const CombineConsts<T>& cmb, int64_t i, int64_t js, double stop, const Records& rt,
+                             int64_t col) {
+  WalkStep<T, 3> w;
+  radial_init<T, CASE>(rc, w);
+  const double tau = mino_tau(js, o.ngeo, stop, false);
+  const T dv = rc.v[R_Xs] * tau;
+  RState<T> rs;
+  WalkState<T, 3> st;
+  T lb[3];
+  radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, rc.v[R_U0] - dv, dv, rs, st, lb);
+  record_store<T, 3>(rt, segment_record(js, kAnchorEvery), kRFields, col, st, lb);
+  return radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, js, tau, rs);
+}
+
+// Segment starts of the block of a repair item (i, js0): js0 + s * L for
+// s = 0, 1, ... while short of the block's end; -1 past it.
+KG_INLINE int64_t repair_segment(int64_t js0, int64_t s, int64_t ngeo, int rcase) {
+  const int64_t L = (rcase == 3 || rcase == 4) ? kAnchorEvery : kAnchorEveryLong;
+  const int64_t last = ngeo - 1;
+  const int64_t jb = js0 / kAnchorBlock * kAnchorBlock;
+  const int64_t je = (jb + kAnchorBlock < last) ? jb + kAnchorBlock : last;
+  const int64_t js = js0 + s * L;
+  return js < je ? js : -1;
 }
 
 // Radial samples j0..j1-1 walked from the records.
@@ -891,6 +936,18 @@ KG_INLINE uint32_t theta_fixup_pixel(const double* __restrict__ plan, int64_t st
   return theta_pixel<DO_PHI_T, T>(plan, stride, p, gflags, o, i, j, fixup_end(j, o.ngeo, walked));
 }
 
+// Queued theta sample j + s (0 <= s < kFixupSpan) of the range j .. fixup_end,
+// by direct evaluation, if it lies in the range: the fixup pass with one
+// sample per lane, whose lanes are independent (the pass is latency-bound:
+// few items, each a chain of dependent evaluations when walked).
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t theta_fixup_sample(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
+                                      const SampleOut& o, int64_t i, int64_t j, bool walked, int64_t s) {
+  const int64_t js = j + s;
+  if (js >= fixup_end(j, o.ngeo, walked)) return 0u;
+  return theta_pixel<DO_PHI_T, T, false>(plan, stride, p, gflags, o, i, js, js + 1);
+}
+
 template <class T>
 KG_INLINE int radial_pick(const double* __restrict__ plan, int64_t stride, int64_t p, RadialConsts<T>& rc, T& lam,
                           double& stop) {
@@ -909,32 +966,40 @@ KG_INLINE bool radial_walks(const SampleOut& o, uint32_t gflags, double m) {
 // A head (FAR = false: direct evaluation) or the long steps of a block
 // (FAR = true) of one radial case, its slots loaded inside the case as in
 // the walk.
-template <class T, int CASE, bool DO_PHI_T, bool FAR>
+// KIND: 0 a head (direct evaluation), 1 the long steps of a block, 2 the
+// repair of a block from segment start k.
+template <class T, int CASE, bool DO_PHI_T, int KIND, class Repair>
 KG_INLINE uint32_t radial_anchor_case(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
-                                      int64_t p, const SampleOut& o, int64_t i, int64_t k, const Records& rt) {
+                                      int64_t p, const SampleOut& o, int64_t i, int64_t k, const Records& rt,
+                                      const Repair& repair) {
   RadialConsts<T> rc;
   T lam;
   double stop;
   radial_pick(plan, stride, p, rc, lam, stop);
   const CombineConsts<T> cmb = combine_consts(c, lam);
-  if (FAR) return radial_far<T, CASE, DO_PHI_T>(o, rc, cmb, i, k, stop, rt, p) ? ST_NONFINITE : 0u;
-  JacobiLadder<T> lad;
-  jacobi_ladder(rc.v[R_k], lad);
-  const bool bad = radial_head<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, k, stop, rt, p);
-  return loop_status(bad, lad);
+  if constexpr (KIND == 1) {
+    return radial_far<T, CASE, DO_PHI_T>(o, rc, cmb, i, k, stop, rt, p, repair) ? ST_NONFINITE : 0u;
+  } else {
+    JacobiLadder<T> lad;
+    jacobi_ladder(rc.v[R_k], lad);
+    bool bad;
+    if constexpr (KIND == 2) bad = radial_repair<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, k, stop, rt, p);
+    else bad = radial_head<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, k, stop, rt, p);
+    return loop_status(bad, lad);
+  }
 }
 
-template <bool DO_PHI_T, bool FAR, class T = double>
+template <bool DO_PHI_T, int KIND, class T = double, class Repair = NoRepair>
 KG_INLINE uint32_t radial_anchor_dispatch(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
                                           int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t k,
-                                          const Records& rt) {
+                                          const Records& rt, const Repair& repair = Repair()) {
   const int rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
   const double m = plan[static_cast<int64_t>(PL_R + R_k) * stride + p];
   if (!radial_walks(o, gflags, m)) return 0u;
   switch (rcase) {
-    case 4: return radial_anchor_case<T, 4, DO_PHI_T, FAR>(c, plan, stride, p, o, i, k, rt);
-    case 3: return radial_anchor_case<T, 3, DO_PHI_T, FAR>(c, plan, stride, p, o, i, k, rt);
-    default: return radial_anchor_case<T, 2, DO_PHI_T, FAR>(c, plan, stride, p, o, i, k, rt);
+    case 4: return radial_anchor_case<T, 4, DO_PHI_T, KIND>(c, plan, stride, p, o, i, k, rt, repair);
+    case 3: return radial_anchor_case<T, 3, DO_PHI_T, KIND>(c, plan, stride, p, o, i, k, rt, repair);
+    default: return radial_anchor_case<T, 2, DO_PHI_T, KIND>(c, plan, stride, p, o, i, k, rt, repair);
   }
 }
 
@@ -943,15 +1008,28 @@ template <bool DO_PHI_T, class T = double>
 KG_INLINE uint32_t radial_head_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
                                      int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t hd,
                                      const Records& rt) {
-  return radial_anchor_dispatch<DO_PHI_T, false, T>(c, plan, stride, p, gflags, o, i, hd, rt);
+  return radial_anchor_dispatch<DO_PHI_T, 0, T>(c, plan, stride, p, gflags, o, i, hd, rt);
 }
 
 // Radial segment starts of block blk of output pixel i after the block's head.
-template <bool DO_PHI_T, class T = double>
+template <bool DO_PHI_T, class T = double, class Repair = NoRepair>
 KG_INLINE uint32_t radial_far_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
                                     int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t blk,
-                                    const Records& rt) {
-  return radial_anchor_dispatch<DO_PHI_T, true, T>(c, plan, stride, p, gflags, o, i, blk, rt);
+                                    const Records& rt, const Repair& repair = Repair()) {
+  return radial_anchor_dispatch<DO_PHI_T, 1, T>(c, plan, stride, p, gflags, o, i, blk, rt, repair);
+}
+
+// Segment s of the repair item (i, js0) of output pixel i: the segment start
+// js0 + s * L of js0's block, if short of the block's end (the repair pass
+// for a block whose far pass lost a long step at js0).
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_repair_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t js0,
+                                       int64_t s, const Records& rt) {
+  const int rcase = static_cast<int>(plan[static_cast<int64_t>(PL_RCASE) * stride + p]);
+  const int64_t js = repair_segment(js0, s, o.ngeo, rcase);
+  if (js < 0) return 0u;
+  return radial_anchor_dispatch<DO_PHI_T, 2, T>(c, plan, stride, p, gflags, o, i, js, rt);
 }
 
 // The walk of one radial case: its plan slots are loaded here, inside the
@@ -993,6 +1071,16 @@ KG_INLINE uint32_t radial_fixup_pixel(const RayScalars<T>& c, const double* __re
                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j,
                                       bool walked) {
   return radial_pixel<DO_PHI_T, T>(c, plan, stride, p, gflags, o, i, j, fixup_end(j, o.ngeo, walked));
+}
+
+// Queued radial sample j + s of the range j .. fixup_end (as theta_fixup_sample).
+template <bool DO_PHI_T, class T = double>
+KG_INLINE uint32_t radial_fixup_sample(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
+                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j,
+                                       bool walked, int64_t s) {
+  const int64_t js = j + s;
+  if (js >= fixup_end(j, o.ngeo, walked)) return 0u;
+  return radial_pixel<DO_PHI_T, T, false>(c, plan, stride, p, gflags, o, i, js, js + 1);
 }
 
 // Per-pixel part of the plan kernels: the plan column, labels and the two

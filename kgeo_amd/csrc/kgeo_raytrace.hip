@@ -89,11 +89,13 @@ __global__ void kg_clear_flags(uint32_t* flags) {
 // 2 count the theta and radial fixup items of one sample pass.
 constexpr int kThetaCount = 1;
 constexpr int kRadialCount = 2;
+constexpr int kRepairCount = 3;  // the radial far pass's repair items
 
 __global__ void kg_clear_counters(uint32_t* flags) {
   if (threadIdx.x == 0) {
     flags[kThetaCount] = 0u;
     flags[kRadialCount] = 0u;
+    flags[kRepairCount] = 0u;
   }
 }
 
@@ -166,6 +168,16 @@ struct DevPush {
   }
 };
 
+// The radial far pass hands a block with a lost long step to the repair
+// pass through the same queue storage (the theta fixups are done with it, and
+// the radial walk's pushes come after the repair pass).  At most one item per
+// block and pixel: within the queue's npix * ceil(ngeo / 8) items.
+struct DevRepair {
+  static constexpr bool kOn = true;
+  FixupQueue q;
+  __device__ void operator()(int64_t i, int64_t j, bool walked) const { DevPush{q}(i, j, walked); }
+};
+
 __device__ inline void unpack_item(unsigned long long e, int64_t& i, int64_t& j, bool& walked) {
   i = static_cast<int64_t>(e & 0xffffffffull);
   j = static_cast<int64_t>((e >> 32) & 0x7fffffffull);
@@ -206,15 +218,17 @@ __global__ __launch_bounds__(kBlock) void kg_theta_walk(DevParams p, SampleArgs 
 }
 
 template <bool DO_PHI_T>
+// one lane per (queued range, sample of the range) (grid-stride)
 __global__ __launch_bounds__(kPassBlock) void kg_theta_fixup(DevParams p, SampleArgs a, FixupQueue q) {
   const unsigned long long c = *q.count;
-  const unsigned long long n = c < q.cap ? c : q.cap;
+  const unsigned long long n = (c < q.cap ? c : q.cap) * kFixupSpan;
   for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kPassBlock + threadIdx.x; e < n;
        e += static_cast<unsigned long long>(gridDim.x) * kPassBlock) {
     int64_t i, j;
     bool walked;
-    unpack_item(q.items[e], i, j, walked);
-    const uint32_t st = theta_fixup_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j, walked);
+    unpack_item(q.items[e / kFixupSpan], i, j, walked);
+    const int64_t sg = static_cast<int64_t>(e % kFixupSpan);
+    const uint32_t st = theta_fixup_sample<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, j, walked, sg);
     if (a.status && st) atomicOr(&a.status[i], st);
   }
 }
@@ -230,13 +244,35 @@ __global__ __launch_bounds__(kPassBlock) void kg_radial_heads(DevParams p, Sampl
 }
 
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kPassBlock) void kg_radial_far(DevParams p, SampleArgs a, Records rt, int64_t nblk) {
+__global__ __launch_bounds__(kPassBlock) void kg_radial_far(DevParams p, SampleArgs a, Records rt, int64_t nblk,
+                                                           FixupQueue qrep) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
+  const DevRepair repair{qrep};
   uint32_t st = 0u;
   for (int64_t b = blockIdx.y; b < nblk; b += gridDim.y)
-    st |= radial_far_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, b, rt);
+    st |= radial_far_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, b, rt, repair);
   if (a.status && st) atomicOr(&a.status[i], st);
+}
+
+// one lane per (repair item, segment of its block) (grid-stride): each of
+// a block's segment starts from the lost one on, by direct evaluation
+constexpr int kRepairSegments = static_cast<int>(kAnchorBlock / kAnchorEvery);
+
+template <bool DO_PHI_T>
+__global__ __launch_bounds__(kPassBlock) void kg_radial_repair(DevParams p, SampleArgs a, Records rt,
+                                                              FixupQueue qrep) {
+  const unsigned long long c = *qrep.count;
+  const unsigned long long n = (c < qrep.cap ? c : qrep.cap) * kRepairSegments;
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kPassBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kPassBlock) {
+    int64_t i, j;
+    bool walked;
+    unpack_item(qrep.items[e / kRepairSegments], i, j, walked);
+    const int64_t sg = static_cast<int64_t>(e % kRepairSegments);
+    const uint32_t st = radial_repair_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j, sg, rt);
+    if (a.status && st) atomicOr(&a.status[i], st);
+  }
 }
 
 template <bool DO_PHI_T>
@@ -251,15 +287,18 @@ __global__ __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(2)))
 }
 
 template <bool DO_PHI_T>
+// one lane per (queued range, sample of the range) (grid-stride)
 __global__ __launch_bounds__(kPassBlock) void kg_radial_fixup(DevParams p, SampleArgs a, FixupQueue q) {
   const unsigned long long c = *q.count;
-  const unsigned long long n = c < q.cap ? c : q.cap;
+  const unsigned long long n = (c < q.cap ? c : q.cap) * kFixupSpan;
   for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kPassBlock + threadIdx.x; e < n;
        e += static_cast<unsigned long long>(gridDim.x) * kPassBlock) {
     int64_t i, j;
     bool walked;
-    unpack_item(q.items[e], i, j, walked);
-    const uint32_t st = radial_fixup_pixel<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j, walked);
+    unpack_item(q.items[e / kFixupSpan], i, j, walked);
+    const int64_t sg = static_cast<int64_t>(e % kFixupSpan);
+    const uint32_t st =
+        radial_fixup_sample<DO_PHI_T>(p.c, a.plan, p.npix, i, a.gflags[0], a.out, i, j, walked, sg);
     if (a.status && st) atomicOr(&a.status[i], st);
   }
 }
@@ -412,6 +451,7 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   unsigned long long* items = reinterpret_cast<unsigned long long*>(base + L.queue);
   const FixupQueue qth{items, flags + kThetaCount, L.qcap};
   const FixupQueue qr{items, flags + kRadialCount, L.qcap};
+  const FixupQueue qrep{items, flags + kRepairCount, L.qcap};
   const unsigned nb = static_cast<unsigned>((d.npix + kPassBlock - 1) / kPassBlock);
   const dim3 wgrid(nb, grid.y);
   const unsigned long long fb = (L.qcap + kPassBlock - 1) / kPassBlock;
@@ -431,8 +471,10 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   if ((rc = check_launch("kg_theta_fixup"))) return rc;
   hipLaunchKernelGGL(kg_radial_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rr, nhead);
   if ((rc = check_launch("kg_radial_heads"))) return rc;
-  hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rr, nblk);
+  hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rr, nblk, qrep);
   if ((rc = check_launch("kg_radial_far"))) return rc;
+  hipLaunchKernelGGL(kg_radial_repair<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, rr, qrep);
+  if ((rc = check_launch("kg_radial_repair"))) return rc;
   hipLaunchKernelGGL(kg_radial_walk<DO_PHI_T>, wgrid, dim3(kPassBlock), 0, st, d, a, rr, qr);
   if ((rc = check_launch("kg_radial_walk"))) return rc;
   hipLaunchKernelGGL(kg_radial_fixup<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, qr);
