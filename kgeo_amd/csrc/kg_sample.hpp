@@ -597,8 +597,8 @@ KG_INLINE bool theta_far(const SampleOut& o, const ThetaConsts<T>& tc, int64_t i
 
 // Theta samples j0..j1-1 walked from the records.  Where a step is refused,
 // `push(i, j, true)` queues samples j.. of the segment for the fixup pass,
-// which evaluates j directly and walks on from there, as the combined loop
-// does; the lane idles until the next segment (j stays uniform).
+// which evaluates each of them directly; the lane idles until the next
+// segment (j stays uniform).
 template <class T, int KIND, bool DO_PHI_T, class Push>
 KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t i, int64_t j0, int64_t j1,
                           double stop, const Records& rt, int64_t col, Push& push) {
@@ -929,7 +929,8 @@ KG_INLINE int64_t fixup_end(int64_t j, int64_t ngeo, bool walked) {
   return je < lim ? je : lim;
 }
 
-// Queued theta samples j .. fixup_end.
+// Queued theta samples j .. fixup_end: the first by direct evaluation, the
+// rest walked from it (the batched survey's fixup pass, one lane per range).
 template <bool DO_PHI_T, class T = double>
 KG_INLINE uint32_t theta_fixup_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
                                      const SampleOut& o, int64_t i, int64_t j, bool walked) {
@@ -1065,7 +1066,7 @@ KG_INLINE uint32_t radial_walk_pixel(const RayScalars<T>& c, const double* __res
   return bad ? ST_NONFINITE : 0u;
 }
 
-// Queued radial samples j .. fixup_end.
+// Queued radial samples j .. fixup_end (as theta_fixup_pixel).
 template <bool DO_PHI_T, class T = double>
 KG_INLINE uint32_t radial_fixup_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
                                       int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j,

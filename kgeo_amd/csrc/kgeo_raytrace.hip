@@ -14,12 +14,16 @@
 //                    ladder, Carlson duplication) at the steps h and L h, the
 //                    last sample and each block's first sample -> records
 //   kg_*_far         one lane per (pixel, block): the block's other segment
-//                    starts by long addition steps -> records
+//                    starts by long addition steps -> records; a radial
+//                    block whose long step is refused (a pole of a
+//                    third-kind integral inside it) is queued for
+//   kg_radial_repair one lane per (queued block, segment): the block's
+//                    remaining segment starts by direct evaluation -> records
 //   kg_*_walk        one lane per (pixel, sample chunk): every other sample
 //                    by the addition theorems from the records; 64 lanes
 //                    write 512 contiguous bytes per output and sample
-//   kg_*_fixup       one lane per queued range: direct evaluation where the
-//                    walk refused a step
+//   kg_*_fixup       one lane per (queued range, sample): direct evaluation
+//                    of the samples where the walk refused a step
 // (kg_sample.hpp).  Caller-given Mino times and grids of at most two samples
 // take kg_theta / kg_radial, one combined loop per coordinate.  The work is
 // FP64 VALU (no MFMA).
@@ -86,7 +90,8 @@ __global__ void kg_clear_flags(uint32_t* flags) {
 }
 
 // Word 0 of the flag page holds the image-wide flags (kg_setup); words 1 and
-// 2 count the theta and radial fixup items of one sample pass.
+// 2 count the theta and radial fixup items of one sample pass, word 3 the
+// radial repair items.
 constexpr int kThetaCount = 1;
 constexpr int kRadialCount = 2;
 constexpr int kRepairCount = 3;  // the radial far pass's repair items
