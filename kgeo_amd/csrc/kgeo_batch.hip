@@ -17,6 +17,9 @@
 //                        counter until the pass is empty; every wave leaves
 //                        once the counter passes the pass's item total, so
 //                        the grid always drains;
+//   kg_batch_repair      after the radial far pass: the blocks whose long
+//                        step was refused, one lane per (block, segment),
+//                        by direct evaluation (grid-stride);
 //   kg_batch_fixup<R>    the queued ranges of all images (grid-stride).
 // Outputs go to caller arenas: image k's [ngeo_k][npix_k] block of r, theta,
 // phi, t starts at element out_off[k] (sum of npix*ngeo of the images before
@@ -63,6 +66,8 @@ enum : int { K_HEAD = 0, K_FAR = 1, K_WALK = 2, K_N = 3 };
 // theta and radial fixup-queue counters
 constexpr int kPassCounters = 2 * K_N;
 constexpr int kQueueCounter = kPassCounters;  // + 0 theta, + 1 r
+constexpr int kRepairCounter = kPassCounters + 2;  // the radial far pass's repair items
+constexpr int kCounters = kPassCounters + 3;
 
 struct BatchImage {
   RayScalars<double> c;
@@ -90,7 +95,7 @@ struct BatchDev {
   unsigned long long qcap;
   uint32_t* flags;    // [nimg]
   uint32_t* done;     // [nimg] completed items
-  uint32_t* counter;  // [kPassCounters + 2]
+  uint32_t* counter;  // [kCounters]
   double *t, *r, *th, *ph;
   int8_t* rclass;
   int8_t* thclass;
@@ -117,7 +122,7 @@ __global__ void kg_batch_clear(BatchDev d) {
     d.flags[k] = 0u;
     d.done[k] = 0u;
   }
-  if (threadIdx.x < kPassCounters + 2) d.counter[threadIdx.x] = 0u;
+  if (threadIdx.x < kCounters) d.counter[threadIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void kg_batch_setup(BatchDev d) {
@@ -164,6 +169,16 @@ struct BatchPush {
   }
 };
 
+// The radial far pass hands a block with a lost long step to kg_batch_repair
+// through the queue storage (the theta fixups are done with it; the radial
+// walk's pushes come after the repair pass): at most one item per block and
+// pixel slot, within the queue's capacity.
+struct BatchRepair {
+  static constexpr bool kOn = true;
+  BatchPush push;
+  __device__ void operator()(int64_t i, int64_t j, bool walked) const { push(i, j, walked); }
+};
+
 __device__ inline SampleOut image_out(const BatchDev& d, const BatchImage& B) {
   const bool phi_t = B.do_phi_t != 0;
   return SampleOut{nullptr, nullptr, phi_t ? d.t + B.out_off : nullptr, d.r + B.out_off, d.th + B.out_off,
@@ -186,8 +201,10 @@ __device__ inline uint32_t batch_unit(const BatchDev& d, const BatchImage& B, in
                  : radial_head_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, u, rt);
   }
   if (KIND == K_FAR) {
+    const BatchRepair repair{
+        BatchPush{d.queue, d.counter + kRepairCounter, d.qcap, static_cast<unsigned long long>(im)}};
     return THETA ? theta_far_pixel<PHI_T>(d.plan, d.total_slots, s, fl, o, i, u, rt)
-                 : radial_far_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, u, rt);
+                 : radial_far_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, u, rt, repair);
   }
   const int64_t j0 = chunk_start(B.ngeo, u, B.units[K_WALK]);
   const int64_t j1 = chunk_start(B.ngeo, u + 1, B.units[K_WALK]);
@@ -252,6 +269,31 @@ __global__ __launch_bounds__(kBlock) void kg_batch_fixup(BatchDev d) {
       st = B.do_phi_t ? radial_fixup_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked)
                       : radial_fixup_pixel<false>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked);
     }
+    if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
+  }
+}
+
+// one lane per (repair item, segment of its block) (grid-stride), as the
+// plain kg_radial_repair
+constexpr int kRepairSegments = static_cast<int>(kAnchorBlock / kAnchorEvery);
+
+__global__ __launch_bounds__(kBlock) void kg_batch_repair(BatchDev d) {
+  const unsigned long long c = d.counter[kRepairCounter];
+  const unsigned long long n = (c < d.qcap ? c : d.qcap) * kRepairSegments;
+  for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
+       e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
+    const unsigned long long q = d.queue[e / kRepairSegments];
+    const int64_t sg = static_cast<int64_t>(e % kRepairSegments);
+    const int im = static_cast<int>((q >> 51) & 0xfffull);
+    const int64_t j = static_cast<int64_t>((q >> 31) & 0xfffffull);
+    const int64_t i = static_cast<int64_t>(q & 0x7fffffffull);
+    const BatchImage& B = d.imgs[im];
+    const int64_t s = B.slot_off + i;
+    const uint32_t fl = d.flags[im];
+    const SampleOut o = image_out(d, B);
+    const Records rt = image_records(d, B, false);
+    const uint32_t st = B.do_phi_t ? radial_repair_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, sg, rt)
+                                   : radial_repair_pixel<false>(B.c, d.plan, d.total_slots, s, fl, o, i, j, sg, rt);
     if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
   }
 }
@@ -474,6 +516,7 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
   hipLaunchKernelGGL(kg_batch_fixup<true>, dim3(blocks), dim3(kBlock), 0, st, d);
   hipLaunchKernelGGL((kg_batch_pass<K_HEAD, false>), dim3(blocks), dim3(kBlock), 0, st, d);
   hipLaunchKernelGGL((kg_batch_pass<K_FAR, false>), dim3(blocks), dim3(kBlock), 0, st, d);
+  hipLaunchKernelGGL(kg_batch_repair, dim3(blocks), dim3(kBlock), 0, st, d);
   hipLaunchKernelGGL(kg_batch_walk<false>, dim3(blocks), dim3(kBlock), 0, st, d);
   hipLaunchKernelGGL(kg_batch_fixup<false>, dim3(blocks), dim3(kBlock), 0, st, d);
   const int lrc = blaunch("batch kernel launch (capture)");
