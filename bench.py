@@ -283,6 +283,32 @@ def bench_equatorial(args):
     print(json.dumps(line), flush=True)
 
 
+def grad_image_rate(steps=20, warmup=3):
+    """The metric's second half: d(image)/d(a, th_o, p) at 256^2 (a = 0.7,
+    th_o = 40 deg, p = 2, mbar 0..2, alpha, beta in [-12, 12]), timed on the
+    host clock around `steps` calls after `warmup`."""
+    import torch
+    from kgeo_amd.equatorial import equatorial_image_vjp_device
+
+    dev = torch.device("cuda", 0)
+    al = np.linspace(-12.0, 12.0, 256)
+    A, B = np.meshgrid(al, al)
+    A = torch.from_numpy(A.flatten()).to(dev)
+    B = torch.from_numpy(B.flatten()).to(dev)
+    th = 40.0 * np.pi / 180.0
+    for _ in range(warmup):
+        equatorial_image_vjp_device(0.7, R_O, th, A, B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        equatorial_image_vjp_device(0.7, R_O, th, A, B)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {"value": 1.0 / dt, "unit": "grad-images/s", "ms_per_grad_image": dt * 1e3, "steps": steps,
+            "workload": "d image / d (a, th_o, p) at 256^2, a=0.7, th_o=40 deg, p=2, mbar 0..2, "
+                        "alpha,beta in [-12,12] (bench.py --mode equatorial for the 1024^2 image)"}
+
+
 def bench_copy(args):
     """A plain device-to-device copy of known size (1 GiB read + 1 GiB
     written per launch): the reference point that scales the FETCH_SIZE and
@@ -398,6 +424,9 @@ def main():
                 "hardware_fp64_flops_per_launch": read_profile_value("hardware_fp64_flops_per_launch"),
                 "ops_per_sample": {k: round(v, 3) for k, v in ops.items()}}
 
+    grad = grad_image_rate()
+    log(f"grad-images: {grad['ms_per_grad_image']:.3f} ms -> {grad['value']:.0f} grad-images/s")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(A, B, th)
@@ -409,7 +438,7 @@ def main():
                                    "alpha,beta in [-8,8]); writes r, theta, phi, t",
                        "npix": npix, "ngeo": NGEO, "parallelism": "single GPU, pixels x sample chunks"},
             "roofline": roofline, "cpu_baseline": cpu,
-            "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+            "gpu_vs_cpu": (value / cpu["value"]) if cpu else None, "grad_images": grad,
             "nonfinite_pixels": nonfinite, "host": platform.node()}
     print(json.dumps(line), flush=True)
 
