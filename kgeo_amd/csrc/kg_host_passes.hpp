@@ -35,9 +35,10 @@ inline void unpack_host_item(uint64_t e, int64_t& i, int64_t& j, bool& walked) {
   walked = (e >> 63) != 0ull;
 }
 
-// st [npix] accumulates the status bits; nfix[2] receives the theta and
+// st [npix] accumulates the status bits; nfix[3] receives the theta and
 // radial queue lengths, or -1 where a pass queued more items than the
-// kernels' queue holds (npix * ceil(ngeo / kFixupSpan)).  The host queue
+// kernels' queue holds (npix * ceil(ngeo / kFixupSpan)), and the number of
+// radial repair items.  The host queue
 // itself is unbounded, so the results stay complete either way.
 template <bool DO_PHI_T, class T>
 void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, const SampleOut& o, int chunks,
@@ -75,6 +76,7 @@ void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, c
   for (int64_t i = 0; i < npix; ++i)
     for (int64_t b = 0; b < nblk; ++b)
       st[i] |= radial_far_pixel<DO_PHI_T, T>(c, plan, npix, i, gflags, o, i, b, rr, repair);
+  nfix[2] = static_cast<int64_t>(repairs.size());
   for (uint64_t e : repairs) {
     int64_t i, j;
     bool w;

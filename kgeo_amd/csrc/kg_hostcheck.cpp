@@ -109,8 +109,9 @@ void kgh_setup(const double* sc, const double* alpha, const double* beta, int64_
 // (kg_sample.hpp) as kg_setup and the sample kernels, with ngeo split into
 // `chunks` contiguous ranges per pixel as the kernels' grid does.
 //   passes = 3: the anchor / walk / fixup passes (the kernels' path for the
-//               linspace grid with ngeo > 2); nfix[2] (optional) receives the
-//               theta and radial fixup-queue lengths;
+//               linspace grid with ngeo > 2); nfix[3] (optional) receives the
+//               theta and radial fixup-queue lengths and the radial repair
+//               items;
 //   passes = 1: one combined loop per coordinate (theta_pixel, radial_pixel:
 //               the kernels' path for caller-given Mino times).
 // Outputs [ngeo][npix]; tau_in (optional, [ngeo][npix]) replaces the linspace
@@ -140,12 +141,13 @@ void kgh_raytrace_passes(const double* sc, const double* alpha, const double* be
     o.ph = nullptr;
   }
   if (passes == 3 && !tau_in && G > 2) {
-    int64_t nf[2] = {0, 0};
+    int64_t nf[3] = {0, 0, 0};
     if (do_phi_t) three_passes<true, double>(c, plan.data(), gflags, o, chunks, st.data(), nf);
     else three_passes<false, double>(c, plan.data(), gflags, o, chunks, st.data(), nf);
     if (nfix) {
       nfix[0] = nf[0];
       nfix[1] = nf[1];
+      nfix[2] = nf[2];
     }
     if (status)
       for (int64_t i = 0; i < npix; ++i) status[i] = st[i];

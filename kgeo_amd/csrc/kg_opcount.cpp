@@ -110,7 +110,7 @@ void kgc_count(const double* sc, const double* alpha, const double* beta, int64_
   if (chunks < 1) chunks = 1;
   std::memset(kgcount::g_ops, 0, sizeof(kgcount::g_ops));
   std::vector<uint32_t> st(static_cast<size_t>(npix), 0u);
-  int64_t nfix[2];
+  int64_t nfix[3];
   three_passes<true, CD>(cc, plan.data(), gflags, o, chunks, st.data(), nfix);
   for (int q = 0; q < kgcount::NOPS; ++q) samp_acc[q] += kgcount::g_ops[q];
   for (int q = 0; q < kgcount::NOPS; ++q) {

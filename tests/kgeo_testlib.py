@@ -173,13 +173,14 @@ class Host:
         """The kernels' per-pixel sample code on the CPU, with ngeo split into
         `chunks` ranges as the kernels' grid does; adds the status bits.
         passes=3: the anchor / walk / fixup passes (the kernels' path on the
-        linspace grid; adds the fixup-queue lengths as "nfix"); passes=1: one
+        linspace grid; adds the theta and radial fixup-queue lengths and the
+        radial repair items as "nfix"); passes=1: one
         combined loop per coordinate."""
         n = len(alpha)
         outs = {k: np.empty((ngeo, n)) for k in ("tau", "t", "r", "th", "ph", "sig")}
         nm = np.empty(n, np.int32)
         st = np.empty(n, np.uint32)
-        nfix = np.zeros(2, np.int64)
+        nfix = np.zeros(3, np.int64)
         ti = None if tau_in is None else self._p(np.ascontiguousarray(tau_in, float))
         self.lib.kgh_raytrace_passes(self._p(self.scalars(a, r_o, th_o)), self._p(np.ascontiguousarray(alpha, float)),
                                      self._p(np.ascontiguousarray(beta, float)), ctypes.c_int64(n),

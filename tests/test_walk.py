@@ -99,6 +99,7 @@ CASES = [
     (5.0e-5, 45.0, 8.0, 24, 80, 1),
     (-0.9, 80.0, 10.0, 40, 400, 7),
     (-0.3, 3.0, 12.0, 40, 30, 1),  # ngeo below one segment: queued ranges end at the last sample
+    (0.082, 62.9, 8.0, 32, 160, 4),  # low spin: many radial blocks go through the repair pass
 ]
 
 
@@ -126,6 +127,18 @@ def test_walking_loops_match_direct_evaluation(case):
         # an order of magnitude inside the 1e-10 parity bar
         assert e[~ill].max() < 2e-11, (k, float(e[~ill].max()))
         assert np.all(e[ill] < 100 * sens[ill] + 2e-11), k
+
+
+def test_repair_pass_runs():
+    """Radial long steps across a pole of a third-kind integral hand their
+    blocks to the repair pass (the accuracy of those samples is checked with
+    the rest in test_walking_loops_match_direct_evaluation)."""
+    host = Host()
+    for a, thd, G in ((0.082, 62.9, 160), (0.94, 17.0, 500)):
+        A, B = grid(8.0, 32)
+        out = host.raytrace_loops(a, 1.0e4, np.deg2rad(thd), np.ascontiguousarray(A), np.ascontiguousarray(B), G,
+                                  chunks=4)
+        assert out["nfix"][2] > 0, (a, thd)
 
 
 PASS_CASES = [
