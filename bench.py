@@ -415,7 +415,7 @@ def main():
     achieved_tf = flops_s * samples / (samp_ms * 1e-3) / 1e12
     bytes_per_launch = 32.0 * samples
     hbm_gbs = bytes_per_launch / (samp_ms * 1e-3) / 1e9
-    roofline = {"bound": "valu-fp64", "kernel": "sample passes (kg_theta_* + kg_radial_*: heads, far, walk, fixup)", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+    roofline = {"bound": "valu-fp64", "kernel": "sample passes (kg_theta_* + kg_radial_*: heads, far, repair, walk, fixup)", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS,
                 "traffic": read_profile_value("traffic_bytes_per_launch"),
                 "flops_per_sample": flops_s, "algorithmic_bytes_per_launch": bytes_per_launch,
