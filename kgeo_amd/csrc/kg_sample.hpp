@@ -418,7 +418,7 @@ KG_INLINE uint32_t radial_pixel(const RayScalars<T>& c, const double* __restrict
 // kAnchorEvery), the last record the sample ngeo - 1.
 constexpr int kThFields = 6;   // s, c, d, D, P, Lg
 constexpr int kRFields = 13;   // s, c, d, D, P[3], Lg[3], Lb[3] (walked R1/S1 logs)
-constexpr int64_t kAnchorBlock = 128;
+constexpr int64_t kAnchorBlock = 256;
 
 KG_INLINE int64_t seg_count(int64_t ngeo, int64_t len) { return (ngeo + len - 1) / len; }
 
