@@ -104,12 +104,12 @@ __global__ void kg_clear_counters(uint32_t* flags) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void kg_setup(DevParams p, const double* __restrict__ alpha,
+__global__ __launch_bounds__(kPassBlock) void kg_setup(DevParams p, const double* __restrict__ alpha,
                                                    const double* __restrict__ beta, double* __restrict__ plan,
                                                    uint32_t* __restrict__ gflags, int8_t* __restrict__ rclass,
                                                    int8_t* __restrict__ thclass, int32_t* __restrict__ nmax_eq,
                                                    double* __restrict__ tau_tot, uint32_t* __restrict__ status) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   SetupOut so{false, false};
   if (i < p.npix)
     so = setup_pixel(p.c, p.ngeo, p.do_phi_t != 0, alpha[i], beta[i], plan, p.npix, i, rclass, thclass, nmax_eq,
@@ -440,8 +440,8 @@ int run_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, 
   hipLaunchKernelGGL(kg_clear_flags, dim3(1), dim3(64), 0, st, gflags);
   int rc = check_launch("kg_clear_flags");
   if (rc) return rc;
-  const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(kg_setup, dim3(nb), dim3(kBlock), 0, st, d, alpha, beta, plan, gflags, rclass, thclass, nmax_eq,
+  const unsigned nb = static_cast<unsigned>((d.npix + kPassBlock - 1) / kPassBlock);
+  hipLaunchKernelGGL(kg_setup, dim3(nb), dim3(kPassBlock), 0, st, d, alpha, beta, plan, gflags, rclass, thclass, nmax_eq,
                      tau_tot, status);
   return check_launch("kg_setup");
 }
