@@ -26,6 +26,7 @@ import json
 import multiprocessing as mp
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -86,47 +87,75 @@ def algorithmic_flops_per_sample(A, B, th, chunks, npix_sample=512, seed=0):
 def _cpu_chunk(args):
     from oracle import kgeo_oracle as O
     a, r_o, th, A, B, G = args
-    os.environ["OMP_NUM_THREADS"] = "1"
     t0 = time.perf_counter()
     O.raytrace_ana(a, r_o, th, A, B, G)
     return time.perf_counter() - t0
 
 
-def cpu_baseline(A, B, th, px_per_core=768, seed=1):
-    """The oracle (the reference algorithm restated in NumPy/SciPy) on a seeded
-    sample of the workload's pixels, 1 process and then P processes."""
-    from oracle import kgeo_oracle as O
-    ncores = min(len(os.sched_getaffinity(0)), 16)
-    rng = np.random.default_rng(seed)
-    idx = rng.choice(len(A), px_per_core * ncores, replace=False)
-    # warm-up + single process
-    O.raytrace_ana(A_SPIN, R_O, th, A[idx[:32]], B[idx[:32]], NGEO)
-    t1 = _cpu_chunk((A_SPIN, R_O, th, A[idx[:px_per_core]], B[idx[:px_per_core]], NGEO))
-    rate1 = px_per_core * NGEO / t1
-    log(f"cpu baseline, 1 process: {px_per_core} px x {NGEO} samples in {t1:.2f} s = {rate1:.3e} samples/s")
-    chunks = [(A_SPIN, R_O, th, A[idx[i * px_per_core:(i + 1) * px_per_core]],
-               B[idx[i * px_per_core:(i + 1) * px_per_core]], NGEO) for i in range(ncores)]
-    ctx = mp.get_context("fork")
-    t0 = time.perf_counter()
-    with ctx.Pool(ncores) as pool:
-        pool.map(_cpu_chunk, chunks)
-    tp = time.perf_counter() - t0
-    rateP = px_per_core * ncores * NGEO / tp
-    log(f"cpu baseline, {ncores} processes: {px_per_core * ncores} px in {tp:.2f} s = {rateP:.3e} samples/s")
+def _cpu_pool_init():
+    for k in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS"):
+        os.environ[k] = "1"
+
+
+def _cpu_time(A, B, th, G, procs, reps):
+    """Median wall time of `reps` runs (after one warm-up) of the oracle over
+    pixels A, B: one process, or `procs` processes over contiguous chunks."""
+    def once(pool):
+        if pool is None:
+            return _cpu_chunk((A_SPIN, R_O, th, A, B, G))
+        parts = np.array_split(np.arange(len(A)), procs)
+        t0 = time.perf_counter()
+        pool.map(_cpu_chunk, [(A_SPIN, R_O, th, A[ix], B[ix], G) for ix in parts])
+        return time.perf_counter() - t0
+    if procs == 1:
+        once(None)
+        return float(np.median([once(None) for _ in range(reps)]))
+    with mp.get_context("fork").Pool(procs, initializer=_cpu_pool_init) as pool:
+        once(pool)
+        return float(np.median([once(pool) for _ in range(reps)]))
+
+
+def cpu_baseline(th):
+    """The oracle (the reference algorithm restated in NumPy/SciPy, with the
+    same scipy.special calls) timed on this host: the 64x64 ngeo=100 image in
+    full (409,600 samples) and the seeded 4096-pixel subset of the 512x512
+    ngeo=500 image (2,048,000 samples, the pixel mix the parity tests use),
+    each in one process and in P processes of one thread; one warm-up, then
+    the median of 3 (one timed run for the single-process subset, about 6 s).
+    P is this process's CPU affinity, capped at 16: a one-GPU box shares its
+    host among GPUs and gives each GPU's job 16 cores, whatever nproc shows."""
+    import scipy
+    _cpu_pool_init()
+    procs = max(1, min(len(os.sched_getaffinity(0)), 16))
+    al = np.linspace(-WINDOW, WINDOW, 64)
+    A1, B1 = np.meshgrid(al, al)
+    A1, B1 = A1.flatten(), B1.flatten()
+    A2, B2 = screen()
+    sub = np.random.default_rng(0).choice(len(A2), 4096, replace=False)   # the parity tests' subset
+    As, Bs = A2[sub], B2[sub]
+    runs = {}
+    for name, (A, B, G) in {"c1_full": (A1, B1, 100), "c2_subset": (As, Bs, NGEO)}.items():
+        for P, reps in ((1, 3 if name == "c1_full" else 1), (procs, 3)):
+            t = _cpu_time(A, B, th, G, P, reps)
+            runs[f"{name}_{P}p"] = {"samples": len(A) * G, "processes": P, "seconds": t,
+                                    "samples_per_s": len(A) * G / t, "reps": reps}
+            log(f"cpu baseline {name}, {P} process(es): {len(A) * G} samples in {t:.2f} s "
+                f"= {len(A) * G / t:.3e} samples/s (median of {reps})")
     model = "unknown"
     try:
-        for ln in open("/proc/cpuinfo"):
-            if ln.startswith("model name"):
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name"):
                 model = ln.split(":", 1)[1].strip()
                 break
-    except OSError:
+    except Exception:
         pass
-    import scipy
-    return {"value": rateP, "unit": "samples/s", "cores": ncores, "kind": "port",
-            "sample": f"{px_per_core * ncores} seeded pixels of the 512x512 grid x ngeo=500 "
-                      f"({px_per_core * ncores * NGEO} samples), {ncores} processes x 1 thread",
-            "single_process_value": rate1, "cpu_model": model, "numpy": np.__version__,
-            "scipy": scipy.__version__}
+    head = runs[f"c2_subset_{procs}p"]
+    return {"value": head["samples_per_s"], "unit": "samples/s", "cores": procs, "kind": "port",
+            "sample": "oracle on the seeded 4096-pixel subset of the 512x512 ngeo=500 image (2,048,000 "
+                      f"samples), {procs} processes x 1 thread, median of 3 after a warm-up",
+            "single_process_value": runs["c2_subset_1p"]["samples_per_s"], "runs": runs,
+            "cpu_model": model, "numpy": np.__version__, "scipy": scipy.__version__}
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kg_sample_passes_512x512_ngeo500_pmc.txt")
@@ -149,10 +178,17 @@ def _events(n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
 
-def bench_batch(args):
+BATCH_CANDIDATES = [(64, 8), (128, 8), (256, 8), (128, 16)]
+
+
+def batch_survey_figures(steps, warmup, retune=False, candidates=None):
     """Batched survey: 256 seeded images (spin, inclination, resolution,
     ngeo) in one persistent graph launch, against the same survey as 256
-    sequential plain kgeo_raytrace_ana calls into the same arenas."""
+    sequential plain kgeo_raytrace_ana calls into the same arenas.  The
+    launch shape comes from the autotuner (its cached choice, else a tuning
+    run over every 8th image).  Returns the figures as a dict; the largest
+    relative deviation between the two runs is taken over every 16th image
+    (labels must be identical, NaN patterns equal)."""
     import torch
     from kgeo_amd import autotune
     from kgeo_amd.batch import KEYS, BatchSurvey, layout, run_sequential, survey_draw, survey_images, survey_size
@@ -170,66 +206,78 @@ def bench_batch(args):
     key = autotune.batch_key(npix, ngeo)
     choice = autotune.lookup(key)
     tune_ms = None
-    if choice is None or args.retune:
+    if choice is None or retune:
         sub = images[::8]
         log(f"autotuning on {len(sub)} of the images (key {key})")
         t0 = time.perf_counter()
-        choice, timings = autotune.tune_batch(sub, key=key, arena=arena, log=log)
+        choice, timings = autotune.tune_batch(sub, key=key, arena=arena, log=log, candidates=candidates)
         tune_ms = (time.perf_counter() - t0) * 1e3
         log(f"autotune choice {choice} in {tune_ms / 1e3:.1f} s")
     sv = BatchSurvey(images, r_o=R_O, tuning=choice, arena=arena)
     log(f"planned: {sv.items} work items, tuning {sv.tuning}")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         sv.run()
     sv.stream.synchronize()
-    ev = _events(2 * args.steps)
+    ev = _events(2 * steps)
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         ev[2 * i].record(sv.stream)
         sv.run()
         ev[2 * i + 1].record(sv.stream)
     sv.stream.synchronize()
-    wall_b = (time.perf_counter() - t0) / args.steps
-    gpu_b = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)])) / 1e3
+    wall_b = (time.perf_counter() - t0) / steps
+    gpu_b = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)])) / 1e3
     done, total = sv.progress()
     log(f"batched: {wall_b * 1e3:.1f} ms/survey (events {gpu_b * 1e3:.1f} ms) -> {samples / wall_b:.3e} samples/s")
-    # keep a few images' batched outputs for the accuracy comparison
-    keep = list(range(0, len(images), 37))
-    saved = {k: [sv.image(i)[key_].cpu() for key_ in KEYS] for k, i in enumerate(keep)}
-    labels_b = {k: getattr(sv, k).cpu() for k in ("rclass", "thclass", "nmax_eq", "status")}
+    # the batched outputs of every 16th image, kept on the device for the comparison
+    keep = list(range(0, len(images), 16))
+    saved = {i: {k: sv.image(i)[k].clone() for k in KEYS} for i in keep}
+    labels_b = {k: getattr(sv, k).clone() for k in ("rclass", "thclass", "nmax_eq", "status")}
     dev_labels = {k: torch.empty_like(getattr(sv, k)) for k in ("rclass", "thclass", "nmax_eq", "tau_tot", "status")}
     ws = run_sequential(images, arena, out_off, r_o=R_O, labels=dev_labels)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         run_sequential(images, arena, out_off, r_o=R_O, workspace=ws, labels=dev_labels)
     torch.cuda.synchronize()
-    wall_s = (time.perf_counter() - t0) / args.steps
+    wall_s = (time.perf_counter() - t0) / steps
     log(f"sequential plain launches: {wall_s * 1e3:.1f} ms/survey -> {samples / wall_s:.3e} samples/s")
-    same = True
-    for k, i in enumerate(keep):
+    worst, nan_same = 0.0, True
+    for i in keep:
         im = sv.image(i)
-        for j, key_ in enumerate(KEYS):
-            a_, b_ = saved[k][j], im[key_].cpu()
-            same &= bool(torch.equal(torch.nan_to_num(a_, nan=1.5e308), torch.nan_to_num(b_, nan=1.5e308)))
-    for k in labels_b:
-        same &= bool(torch.equal(labels_b[k], dev_labels[k].cpu()))
+        for k in KEYS:
+            b, p_ = saved[i][k], im[k]
+            nan_same &= bool(torch.equal(torch.isnan(b), torch.isnan(p_)))
+            ok = ~torch.isnan(p_)
+            if ok.any():
+                worst = max(worst, float(((b[ok] - p_[ok]).abs() / p_[ok].abs().clamp(min=1.0)).max()))
+    labels_same = all(bool(torch.equal(labels_b[k], dev_labels[k])) for k in labels_b)
+    log(f"batched vs sequential on {len(keep)} images: largest deviation {worst:.2e}, labels identical "
+        f"{labels_same}, NaN patterns equal {nan_same}")
+    out = {"ms_per_survey": wall_b * 1e3, "samples_per_s": samples / wall_b, "event_ms": gpu_b * 1e3,
+           "sequential_ms_per_survey": wall_s * 1e3, "sequential_samples_per_s": samples / wall_s,
+           "speedup_vs_sequential": wall_s / wall_b, "largest_deviation_vs_sequential": worst,
+           "compared_images": len(keep), "labels_identical": labels_same, "nan_patterns_equal": nan_same,
+           "all_items_done": bool(np.array_equal(done, total)), "images": len(images), "pixels": int(pix),
+           "samples": int(samples), "tuning": {"samples_per_item": sv.tuning[0], "waves_per_cu": sv.tuning[1]},
+           "work_items": sv.items, "autotune_key": key, "autotune_ms": tune_ms, "steps": steps}
+    sv.close()
+    del sv, saved, arena, ws
+    torch.cuda.empty_cache()
+    return out
+
+
+BATCH_WORKLOAD = ("256-image survey: a~U(-0.99,0.99), th_o~U(1,89) deg, res from {64,128,256,512}^2, "
+                  "ngeo~U{50..400}, default_rng(0); alpha,beta in [-8,8], r_o=1e4; writes r, theta, phi, t")
+
+
+def bench_batch(args):
+    f = batch_survey_figures(args.steps, args.warmup, retune=args.retune)
     line = {"metric": "batched survey geodesic samples/s (256 images, one persistent hipGraph launch)",
-            "value": samples / wall_b, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": wall_b * 1e3, "higher_is_better": True, "scaling": "weak",
+            "value": f["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": f["ms_per_survey"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "256-image survey: a~U(-0.99,0.99), th_o~U(1,89) deg, res from "
-                                   "{64,128,256,512}^2, ngeo~U{50..400}, default_rng(0); alpha,beta in [-8,8], "
-                                   "r_o=1e4; writes r, theta, phi, t",
-                       "images": len(images), "pixels": int(pix), "samples": int(samples),
-                       "tuning": {"samples_per_item": sv.tuning[0], "waves_per_cu": sv.tuning[1]},
-                       "work_items": sv.items, "autotune_key": key, "autotune_ms": tune_ms},
-            "batched_event_ms": gpu_b * 1e3,
-            "sequential_plain": {"value": samples / wall_s, "ms_per_survey": wall_s * 1e3,
-                                 "launches": 4 * len(images)},
-            "speedup_vs_sequential": wall_s / wall_b,
-            "outputs_identical_to_plain": same,
-            "all_items_done": bool(np.array_equal(done, total))}
+            "config": {"workload": BATCH_WORKLOAD}, "batch": f}
     print(json.dumps(line), flush=True)
 
 
@@ -337,6 +385,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="leave the batched-survey figures out of the line")
     ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial", "copy"), default="raytrace")
     ap.add_argument("--retune", action="store_true", help="batch mode: re-run the autotuner")
     args = ap.parse_args()
@@ -429,7 +478,14 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(A, B, th)
+        cpu = cpu_baseline(th)
+
+    batch = None
+    if not args.no_batch:
+        del out, ws
+        torch.cuda.empty_cache()
+        batch = batch_survey_figures(2, 1, candidates=BATCH_CANDIDATES)
+        batch["workload"] = BATCH_WORKLOAD
 
     line = {"metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -438,7 +494,7 @@ def main():
                                    "alpha,beta in [-8,8]); writes r, theta, phi, t",
                        "npix": npix, "ngeo": NGEO, "parallelism": "single GPU, pixels x sample chunks"},
             "roofline": roofline, "cpu_baseline": cpu,
-            "gpu_vs_cpu": (value / cpu["value"]) if cpu else None, "grad_images": grad,
+            "gpu_vs_cpu": (value / cpu["value"]) if cpu else None, "grad_images": grad, "batch": batch,
             "nonfinite_pixels": nonfinite, "host": platform.node()}
     print(json.dumps(line), flush=True)
 

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define KGEO_ABI_VERSION 2
+#define KGEO_ABI_VERSION 3
 
 /* return codes */
 #define KGEO_OK 0
@@ -35,7 +35,11 @@ extern "C" {
 #define KGEO_E_WORKSPACE (-3) /* workspace too small or misaligned */
 
 /* params.flags */
-#define KGEO_DO_PHI_T 1u /* compute t, phi, sigma (the reference always does) */
+#define KGEO_DO_PHI_T 1u /* sample t, phi, sigma: required by kgeo_plan, kgeo_sample,
+                            kgeo_raytrace_ana, kgeo_coords_at_tau and every image of
+                            kgeo_batch_plan (the reference's raytrace_ana always samples
+                            them, kerr_raytracing_ana.py:97); calls without it return
+                            KGEO_E_ARG */
 
 /* per-pixel status bits */
 #define KGEO_ST_NONFINITE 1u         /* some output sample is NaN or inf */
@@ -83,10 +87,10 @@ size_t kgeo_workspace_bytes(int64_t npix, int64_t ngeo);
  * at every grid point (coords_at_tau, :126-193).
  *   alpha, beta      [npix]        screen coordinates
  *   tau              [ngeo][npix]  Geodesics.mino_times
- *   t, r, th, ph     [ngeo][npix]  Geodesics.geo_coords; t and ph stay
- *                                   untouched when KGEO_DO_PHI_T is clear
+ *   t, r, th, ph     [ngeo][npix]  Geodesics.geo_coords (t and ph are required:
+ *                                   the theta pass parks G_phi, G_t in them)
  *   sig              [ngeo][npix]  Geodesics.affine_times
- *   (every output pointer may be NULL to skip that output)
+ *   (tau, r, th and sig may be NULL to skip that output)
  *   rclass           [npix] int8   optional: radial_roots rclass (utils.py:366-378)
  *   thclass          [npix] int8   optional: angular_turning thclass (utils.py:266-272)
  *   nmax_eq          [npix] int32  optional: Geodesics.nmax_eq (utils.py:63-67)
@@ -145,6 +149,27 @@ int kgeo_crossings(const kgeo_ray_params* p, const double* alpha, const double* 
                    int64_t ntau, double* n_equatorial, double* n_poloidal, void* stream);
 
 /*
+ * Replaces n_angular_turnings (kerr_raytracing_utils.py:686-792) and adds
+ * the radial turning count the reference decides inline
+ * (equatorial_images.py:266-288), at Mino times tau [ntau][npix]:
+ *   n_angular   [ntau][npix] float64  n_angular_turnings as the reference
+ *                                     returns it (fractional; eta <= 0 rays
+ *                                     take its vortical formula, :753-786)
+ *   turns_theta [ntau][npix] int32    floor(n_angular): theta turning points
+ *                                     passed by tau (INT32_MIN where n_angular
+ *                                     is NaN, i.e. eta == 0)
+ *   turns_r     [ntau][npix] int8     radial turning points passed by tau: 1
+ *                                     where the ray is outside the critical
+ *                                     curve (is_outside_crit, :794-826) and
+ *                                     tau > tau_tot / 2 (mino_total, :382-548,
+ *                                     with raytrace_ana's spin-zero eta fix), else 0
+ * Any output may be NULL.  n_angular and turns_theta need th_o in [0, pi/2],
+ * as the reference's n_angular_turnings does (:690-691; KGEO_E_ARG otherwise).
+ */
+int kgeo_turnings(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau,
+                  int64_t ntau, double* n_angular, int32_t* turns_theta, int8_t* turns_r, void* stream);
+
+/*
  * Replaces the elliptic plug-in ellippi_arr(n, phi, m)
  * (scipy_ellip_binding.py:87-237; GSL twin gsl_ellip_binding.py:154-237):
  * elementwise Pi(n; phi | m) in the A&S convention with the same special
@@ -198,8 +223,8 @@ const char* kgeo_equatorial_last_error(void);
  *     per-pixel arrays, image k at pix_off[k];
  *   t, r, th, ph: one arena, image k's [ngeo_k][npix_k] block at out_off[k].
  * tau and sigma are not stored (tau is the image's linspace grid; sigma can
- * be recomputed with kgeo_coords_at_tau).  t and ph may be NULL only if no
- * image sets KGEO_DO_PHI_T; rclass..status are optional.
+ * be recomputed with kgeo_coords_at_tau).  Every image must set
+ * KGEO_DO_PHI_T and t, ph are required; rclass..status are optional.
  *
  * kgeo_batch_plan uploads the image descriptors, records the graph on
  * `stream` (must not be NULL) and returns a handle; it runs no kernels.

@@ -12,7 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # must equal KGEO_ABI_VERSION of include/kgeo.h
-ABI_VERSION = 2
+ABI_VERSION = 3
 LIB_PATH = os.path.join(_HERE, "libkgeo_hip.so")
 
 KGEO_DO_PHI_T = 1
@@ -78,6 +78,8 @@ _PROTOS = {
     "kgeo_equatorial_last_error": (ctypes.c_char_p, []),
     "kgeo_crossings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
                        + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "kgeo_turnings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
+                      + [ctypes.c_int64] + [ctypes.c_void_p] * 4),
     "kgeo_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(RayParams), ctypes.c_int32]),
     "kgeo_batch_layout": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_void_p]),

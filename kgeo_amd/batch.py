@@ -61,24 +61,27 @@ def survey_size(images):
     return pix, smp
 
 
-def _phi_t(im, default):
-    """An image's do_phi_and_t: its own "do_phi_and_t" entry, else the survey's."""
-    return bool(im.get("do_phi_and_t", default))
+def _check_phi_t(images, default=True):
+    """Every image samples t, phi and sigma (as raytrace_ana does,
+    kerr_raytracing_ana.py:97); a survey or image asking otherwise is refused."""
+    for k, im in enumerate(images):
+        if not bool(im.get("do_phi_and_t", default)):
+            raise _lib.KgeoError(f"image {k}: do_phi_and_t=False is not supported by the batched survey "
+                                 "(t, phi and sigma are always sampled)")
 
 
-def _params_array(images, r_o, maxtaufrac, do_phi_and_t):
+def _params_array(images, r_o, maxtaufrac):
     arr = (_lib.RayParams * len(images))()
     for k, im in enumerate(images):
-        arr[k] = _lib.make_params(im["a"], r_o, im["th_o"], len(im["alpha"]), int(im["ngeo"]), maxtaufrac,
-                                  _phi_t(im, do_phi_and_t))
+        arr[k] = _lib.make_params(im["a"], r_o, im["th_o"], len(im["alpha"]), int(im["ngeo"]), maxtaufrac)
     return arr
 
 
-def layout(images, r_o=1.0e4, maxtaufrac=MAXTAUFRAC, do_phi_and_t=True):
+def layout(images, r_o=1.0e4, maxtaufrac=MAXTAUFRAC):
     """(pix_off, out_off) int64 arrays of nimg + 1 entries (kgeo_batch_layout)."""
     lib = _lib.load()
     n = len(images)
-    params = _params_array(images, r_o, maxtaufrac, do_phi_and_t)
+    params = _params_array(images, r_o, maxtaufrac)
     pix = np.zeros(n + 1, dtype=np.int64)
     out = np.zeros(n + 1, dtype=np.int64)
     rc = lib.kgeo_batch_layout(params, n, pix.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
@@ -90,8 +93,8 @@ class BatchSurvey:
     """A planned survey: device arenas, workspace and the captured graph.
 
     images: sequence of dicts with a, th_o, ngeo, alpha, beta (arrays or
-    tensors), and optionally do_phi_and_t, which overrides the survey's
-    for that image (t and ph of an image without it are left unwritten).
+    tensors).  Every image samples t, phi and sigma; do_phi_and_t=False (for
+    the survey or in an image's dict) raises KgeoError.
     tuning: (samples_per_item, waves_per_cu), or None to use the
     autotuner's cached choice for this survey's shape (default (64, 8) when
     nothing is cached).  ``arena`` may pass preallocated flat float64 tensors
@@ -107,8 +110,9 @@ class BatchSurvey:
         if n == 0:
             raise ValueError("empty survey")
         self.r_o = float(r_o)
-        self.params = _params_array(self.images, self.r_o, maxtaufrac, do_phi_and_t)
-        self.pix_off, self.out_off = layout(self.images, self.r_o, maxtaufrac, do_phi_and_t)
+        _check_phi_t(self.images, do_phi_and_t)
+        self.params = _params_array(self.images, self.r_o, maxtaufrac)
+        self.pix_off, self.out_off = layout(self.images, self.r_o, maxtaufrac)
         self.npix = np.array([len(im["alpha"]) for im in self.images], dtype=np.int64)
         self.ngeo = np.array([int(im["ngeo"]) for im in self.images], dtype=np.int64)
         dev = self.dev
@@ -118,11 +122,7 @@ class BatchSurvey:
         self.beta = torch.cat([_as_device_f64(im["beta"], dev).reshape(-1) for im in self.images])
         arena = dict(arena or {})
         self.arena = {}
-        any_phi_t = any(_phi_t(im, do_phi_and_t) for im in self.images)
         for k in KEYS:
-            if not any_phi_t and k in ("t", "ph"):
-                self.arena[k] = None
-                continue
             a = arena.get(k)
             if a is None:
                 a = torch.empty(total_out, device=dev, dtype=torch.float64)
@@ -167,11 +167,22 @@ class BatchSurvey:
         return int(self.out_off[-1])
 
     def run(self, stream=None):
-        """Replay the survey graph (asynchronous) on `stream` (default: the
-        survey's own stream)."""
+        """Replay the survey graph on `stream` (default: the survey's own
+        stream), ordered after the work already queued on torch's current
+        stream (writes to alpha, beta or the arenas) and before any work queued
+        on it afterwards (reads of the arenas through image()).  Returns the
+        stream the graph ran on."""
         s = stream if stream is not None else self.stream
+        cur = torch.cuda.current_stream(self.dev)
+        if s != cur:
+            s.wait_stream(cur)
         rc = self.lib.kgeo_batch_run(self.handle, ctypes_void(s.cuda_stream))
         _lib.check(rc, "kgeo_batch_run")
+        if s != cur:
+            # the caching allocator must not hand these to other work before the replay ends
+            for t in [self.workspace, self.alpha, self.beta] + [a for a in self.arena.values() if a is not None]:
+                t.record_stream(s)
+            cur.wait_stream(s)
         return s
 
     def progress(self):
@@ -209,8 +220,7 @@ class BatchSurvey:
             pass
 
 
-def run_sequential(images, arena, out_off, r_o=1.0e4, maxtaufrac=MAXTAUFRAC, do_phi_and_t=True, workspace=None,
-                   labels=None):
+def run_sequential(images, arena, out_off, r_o=1.0e4, maxtaufrac=MAXTAUFRAC, workspace=None, labels=None):
     """The same survey as len(images) plain kgeo_raytrace_ana calls, one per
     image, writing into the same arena layout (tau and sigma not stored).
     Asynchronous on torch's current stream."""
@@ -227,15 +237,12 @@ def run_sequential(images, arena, out_off, r_o=1.0e4, maxtaufrac=MAXTAUFRAC, do_
     pix0 = 0
     for k, im in enumerate(images):
         P, G = len(im["alpha"]), int(im["ngeo"])
-        phi_t = _phi_t(im, do_phi_and_t)
-        p = _lib.make_params(im["a"], r_o, im["th_o"], P, G, maxtaufrac, phi_t)
+        p = _lib.make_params(im["a"], r_o, im["th_o"], P, G, maxtaufrac)
         o0 = int(out_off[k])
 
         def view(key):
             a = arena.get(key)
-            if a is None or (not phi_t and key in ("t", "ph")):
-                return None
-            return a[o0:o0 + P * G]
+            return None if a is None else a[o0:o0 + P * G]
 
         def lab(key):
             a = labels.get(key)

@@ -9,9 +9,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <climits>
 #include <cstring>
 #include <vector>
 
+#include "kg_counts.hpp"
 #include "kg_dual.hpp"
 #include "kg_equatorial.hpp"
 #include "kg_geodesic.hpp"
@@ -183,6 +185,28 @@ void kgh_raytrace(const double* sc, const double* alpha, const double* beta, int
                   const double* tau_in, double* tau, double* t, double* r, double* th, double* ph, double* sig,
                   int32_t* nmax_eq) {
   kgh_raytrace_loops(sc, alpha, beta, npix, ngeo, do_phi_t, tau_in, 1, tau, t, r, th, ph, sig, nmax_eq, nullptr);
+}
+
+// Count formulas of the kg_counts / kg_turnings kernels (kg_counts.hpp) at
+// Mino times tau [ntau][n]; every output [ntau][n] (any may be NULL).
+void kgh_counts(const double* sc, const double* alpha, const double* beta, int64_t n, const double* tau,
+                int64_t ntau, double* n_eq, double* n_pol, double* n_ang, int32_t* turns_th, int8_t* turns_r) {
+  RayScalars<double> c = scalars_from(sc);
+  const bool north = sc[2] < kHalfPi;
+  for (int64_t i = 0; i < n; ++i) {
+    const CountConsts k = count_consts(c, alpha[i], beta[i]);
+    const RadialTurn rt = radial_turn_consts(c, alpha[i], beta[i]);
+    for (int64_t j = 0; j < ntau; ++j) {
+      const int64_t o = j * n + i;
+      const double tj = tau[o];
+      if (n_eq) n_eq[o] = n_equatorial_at(k, north, beta[i], tj);
+      if (n_pol) n_pol[o] = n_poloidal_at(k, tj);
+      const double na = n_angular_at(k, beta[i], tj);
+      if (n_ang) n_ang[o] = na;
+      if (turns_th) turns_th[o] = (na == na && std::fabs(na) < 2.0e9) ? static_cast<int32_t>(std::floor(na)) : INT32_MIN;
+      if (turns_r) turns_r[o] = static_cast<int8_t>(radial_turns_at(rt, tj));
+    }
+  }
 }
 
 // Equatorial image (kg_equatorial.hpp) on the CPU: per-subring arrays are

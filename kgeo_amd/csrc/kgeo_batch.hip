@@ -20,7 +20,8 @@
 //   kg_batch_repair      after the radial far pass: the blocks whose long
 //                        step was refused, one lane per (block, segment),
 //                        by direct evaluation (grid-stride);
-//   kg_batch_fixup<R>    the queued ranges of all images (grid-stride).
+//   kg_batch_fixup<R>    the queued ranges of all images, one lane per
+//                        (range, sample), by direct evaluation (grid-stride).
 // Outputs go to caller arenas: image k's [ngeo_k][npix_k] block of r, theta,
 // phi, t starts at element out_off[k] (sum of npix*ngeo of the images before
 // it); per-pixel inputs and labels are concatenated at pix_off[k].
@@ -180,22 +181,24 @@ struct BatchRepair {
 };
 
 __device__ inline SampleOut image_out(const BatchDev& d, const BatchImage& B) {
-  const bool phi_t = B.do_phi_t != 0;
-  return SampleOut{nullptr, nullptr, phi_t ? d.t + B.out_off : nullptr, d.r + B.out_off, d.th + B.out_off,
-                   phi_t ? d.ph + B.out_off : nullptr, nullptr, B.npix, B.ngeo};
+  return SampleOut{nullptr, nullptr, d.t + B.out_off, d.r + B.out_off, d.th + B.out_off, d.ph + B.out_off, nullptr,
+                   B.npix, B.ngeo};
 }
 
 __device__ inline Records image_records(const BatchDev& d, const BatchImage& B, bool theta) {
   return Records{d.rec + (theta ? B.th_rec : B.r_rec), B.slots, B.slot_off};
 }
 
-// One unit of pass KIND (theta if THETA) for pixel i of image B.
-template <int KIND, bool THETA, bool PHI_T>
+// One unit of pass KIND (theta if THETA) for pixel i of image B.  Every
+// image samples t, phi and sigma (the plan rejects images without
+// KGEO_DO_PHI_T), so PHI_T is true throughout.
+template <int KIND, bool THETA>
 __device__ inline uint32_t batch_unit(const BatchDev& d, const BatchImage& B, int im, int64_t i, int64_t u) {
   const int64_t s = B.slot_off + i;
   const uint32_t fl = d.flags[im];
   const SampleOut o = image_out(d, B);
   const Records rt = image_records(d, B, THETA);
+  constexpr bool PHI_T = true;
   if (KIND == K_HEAD) {
     return THETA ? theta_head_pixel<PHI_T>(d.plan, d.total_slots, s, fl, o, i, u, rt)
                  : radial_head_pixel<PHI_T>(B.c, d.plan, d.total_slots, s, fl, o, i, u, rt);
@@ -227,8 +230,7 @@ __device__ inline void batch_pass(const BatchDev& d) {
     const int64_t i = (li / units) * kWave + lane;
     const int64_t u = li % units;
     if (i < B.npix) {
-      const uint32_t st = B.do_phi_t ? batch_unit<KIND, THETA, true>(d, B, im, i, u)
-                                     : batch_unit<KIND, THETA, false>(d, B, im, i, u);
+      const uint32_t st = batch_unit<KIND, THETA>(d, B, im, i, u);
       if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
     }
     atomicAdd(&d.done[im], lane == 0 ? 1u : 0u);
@@ -246,13 +248,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   batch_pass<K_WALK, THETA>(d);
 }
 
+// one lane per (queued range, sample of the range) (grid-stride), each
+// sample by direct evaluation, as the plain kg_theta_fixup / kg_radial_fixup
 template <bool THETA>
 __global__ __launch_bounds__(kBlock) void kg_batch_fixup(BatchDev d) {
   const unsigned long long c = d.counter[kQueueCounter + (THETA ? 0 : 1)];
-  const unsigned long long n = c < d.qcap ? c : d.qcap;
+  const unsigned long long n = (c < d.qcap ? c : d.qcap) * kFixupSpan;
   for (unsigned long long e = static_cast<unsigned long long>(blockIdx.x) * kBlock + threadIdx.x; e < n;
        e += static_cast<unsigned long long>(gridDim.x) * kBlock) {
-    const unsigned long long q = d.queue[e];
+    const unsigned long long q = d.queue[e / kFixupSpan];
+    const int64_t sg = static_cast<int64_t>(e % kFixupSpan);
     const int im = static_cast<int>((q >> 51) & 0xfffull);
     const int64_t j = static_cast<int64_t>((q >> 31) & 0xfffffull);
     const int64_t i = static_cast<int64_t>(q & 0x7fffffffull);
@@ -261,14 +266,8 @@ __global__ __launch_bounds__(kBlock) void kg_batch_fixup(BatchDev d) {
     const int64_t s = B.slot_off + i;
     const uint32_t fl = d.flags[im];
     const SampleOut o = image_out(d, B);
-    uint32_t st;
-    if (THETA) {
-      st = B.do_phi_t ? theta_fixup_pixel<true>(d.plan, d.total_slots, s, fl, o, i, j, walked)
-                      : theta_fixup_pixel<false>(d.plan, d.total_slots, s, fl, o, i, j, walked);
-    } else {
-      st = B.do_phi_t ? radial_fixup_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked)
-                      : radial_fixup_pixel<false>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked);
-    }
+    const uint32_t st = THETA ? theta_fixup_sample<true>(d.plan, d.total_slots, s, fl, o, i, j, walked, sg)
+                              : radial_fixup_sample<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, walked, sg);
     if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
   }
 }
@@ -292,8 +291,7 @@ __global__ __launch_bounds__(kBlock) void kg_batch_repair(BatchDev d) {
     const uint32_t fl = d.flags[im];
     const SampleOut o = image_out(d, B);
     const Records rt = image_records(d, B, false);
-    const uint32_t st = B.do_phi_t ? radial_repair_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, sg, rt)
-                                   : radial_repair_pixel<false>(B.c, d.plan, d.total_slots, s, fl, o, i, j, sg, rt);
+    const uint32_t st = radial_repair_pixel<true>(B.c, d.plan, d.total_slots, s, fl, o, i, j, sg, rt);
     if (d.status && st) atomicOr(&d.status[B.pix_off + i], st);
   }
 }
@@ -317,6 +315,8 @@ int check_images(const kgeo_ray_params* imgs, int32_t nimg) {
     if (imgs[k].npix >= (1LL << 31)) return bfail(KGEO_E_ARG, "image %lld has npix >= 2^31", k);
     if (imgs[k].ngeo < 1) return bfail(KGEO_E_ARG, "image %lld has ngeo < 1", k);
     if (imgs[k].ngeo > (1LL << 20)) return bfail(KGEO_E_ARG, "image %lld has ngeo > 2^20", k);
+    // every image samples t, phi and sigma, as the plain kernels do
+    if (!(imgs[k].flags & KGEO_DO_PHI_T)) return bfail(KGEO_E_ARG, "image %lld lacks KGEO_DO_PHI_T", k);
   }
   return KGEO_OK;
 }
@@ -415,7 +415,6 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
   std::vector<uint32_t> per(nimg);
   int64_t pix = 0, slots = 0, outo = 0, rec = 0;
   int64_t items[K_N] = {0, 0, 0};
-  bool any_phi = false;
   // walk items hold whole segments (chunk starts are multiples of kAnchorEveryLong)
   const int64_t spi =
       (static_cast<int64_t>(tn.samples_per_item) + kAnchorEveryLong - 1) / kAnchorEveryLong * kAnchorEveryLong;
@@ -436,8 +435,7 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
     rec += theta_rec_elems(p);
     B.r_rec = rec;
     rec += radial_rec_elems(p);
-    B.do_phi_t = (p.flags & KGEO_DO_PHI_T) ? 1 : 0;
-    any_phi |= B.do_phi_t != 0;
+    B.do_phi_t = 1;  // required by check_images
     const int64_t tiles = B.slots / kWave;
     int64_t mine = 0;
     for (int q = 0; q < K_N; ++q) {
@@ -451,7 +449,7 @@ int kgeo_batch_plan(const kgeo_ray_params* imgs, int32_t nimg, const kgeo_batch_
     outo += p.npix * p.ngeo;
     per[k] = static_cast<uint32_t>(2 * mine);  // theta and radial passes
   }
-  if (any_phi && (!t || !ph)) return bfail(KGEO_E_ARG, "t and ph are required with KGEO_DO_PHI_T");
+  if (!t || !ph) return bfail(KGEO_E_ARG, "t and ph are required");
 
   char* w = static_cast<char*>(workspace);
   BatchDev d;

@@ -36,6 +36,7 @@
 #include <cstring>
 
 #include "../../include/kgeo.h"
+#include "kg_counts.hpp"
 #include "kg_sample.hpp"
 
 using namespace kg;
@@ -67,6 +68,7 @@ struct DevParams {
   RayScalars<double> c;
   int64_t npix, ngeo;
   int do_phi_t;
+  int north;  // th_o < pi/2 (n_equatorial_crossings' branch test)
 };
 
 DevParams make_dev(const kgeo_ray_params* p, int64_t ngeo) {
@@ -81,6 +83,7 @@ DevParams make_dev(const kgeo_ray_params* p, int64_t ngeo) {
   d.npix = p->npix;
   d.ngeo = ngeo;
   d.do_phi_t = (p->flags & KGEO_DO_PHI_T) ? 1 : 0;
+  d.north = (p->th_o < kHalfPi) ? 1 : 0;
   return d;
 }
 
@@ -322,49 +325,50 @@ __global__ __launch_bounds__(kBlock) void kg_ellip_pi(const double* __restrict__
   out[i] = (kfabs(phi[i]) <= 1.0e-10) ? phi[i] : ellip_pi_amp(L, nn, mm, Pc);
 }
 
-// Per-sample integer / fractional counts (kerr_raytracing_utils.py:551-684).
+// Per-sample counts (kerr_raytracing_utils.py:551-684): equatorial crossings
+// and poloidal orbits, one lane per pixel (kg_counts.hpp).
 __global__ __launch_bounds__(kBlock) void kg_counts(DevParams p, const double* __restrict__ alpha,
                                                     const double* __restrict__ beta, const double* __restrict__ tau,
                                                     int64_t ntau, double* __restrict__ n_eq,
                                                     double* __restrict__ n_pol) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= p.npix) return;
-  const RayScalars<double>& c = p.c;
-  const double al = alpha[i], be = beta[i];
-  const double lam = -al * c.sin_th;
-  const double eta = (al * al - c.a2) * c.cos2_th + be * be;
-  const bool vort = eta <= 0.0;
-  double up = 0.0, um = 0.0, ratio = 0.0, a2um = 0.0, F0 = 0.0, Kinc = 0.0, K = 0.0, w = 0.0;
-  if (!vort) {
-    uplus_uminus(c, lam, eta, up, um);
-    if (c.minspin) {
-      ratio = 0.0 * um;
-      a2um = -(eta + lam * lam);
-    } else {
-      ratio = up / um;
-      a2um = c.a2 * um;
-    }
-    K = ellip_K(ratio);
-    Kinc = K;
-    F0 = legendre_F(legendre_base(amp_from_sin(c.cos_th / ksqrt(up)), ratio), K);
-    w = ksqrt(-a2um);
-  }
+  const double be = beta[i];
+  const CountConsts k = count_consts(p.c, alpha[i], be);
   for (int64_t j = 0; j < ntau; ++j) {
     const int64_t o = j * p.npix + i;
     const double tj = tau[o];
-    if (vort) {
-      if (n_eq) n_eq[o] = -2.0;
-      if (n_pol) n_pol[o] = -2.0;
-      continue;
+    if (n_eq) n_eq[o] = n_equatorial_at(k, p.north != 0, be, tj);
+    if (n_pol) n_pol[o] = n_poloidal_at(k, tj);
+  }
+}
+
+// Turning points passed by each Mino time (kerr_raytracing_utils.py:686-826),
+// one lane per pixel: the reference's fractional n_angular_turnings, its
+// floor (theta turning points) and the radial turning point (kg_counts.hpp).
+__global__ __launch_bounds__(kBlock) void kg_turnings(DevParams p, const double* __restrict__ alpha,
+                                                      const double* __restrict__ beta,
+                                                      const double* __restrict__ tau, int64_t ntau,
+                                                      double* __restrict__ n_ang, int32_t* __restrict__ turns_th,
+                                                      int8_t* __restrict__ turns_r) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= p.npix) return;
+  const double al = alpha[i], be = beta[i];
+  const bool ang = n_ang || turns_th;
+  CountConsts k{};
+  if (ang) k = count_consts(p.c, al, be);
+  RadialTurn rt{false, 0.0};
+  if (turns_r) rt = radial_turn_consts(p.c, al, be);
+  for (int64_t j = 0; j < ntau; ++j) {
+    const int64_t o = j * p.npix + i;
+    const double tj = tau[o];
+    if (ang) {
+      const double n = n_angular_at(k, be, tj);
+      if (n_ang) n_ang[o] = n;
+      // NaN (eta == 0 rays) maps to INT32_MIN, as numpy's float -> int32 cast does on x86
+      if (turns_th) turns_th[o] = (n == n && kfabs(n) < 2.0e9) ? static_cast<int32_t>(kfloor(n)) : INT32_MIN;
     }
-    const double x = tj * w;
-    if (n_eq) {
-      double n;
-      if (c.h > 0.0) n = (be <= 0.0) ? kfloor((x - F0) / (2.0 * Kinc)) : kfloor((x + F0) / (2.0 * Kinc)) - 1.0;
-      else n = (be >= 0.0) ? kfloor((x + F0) / (2.0 * Kinc)) : kfloor((x - F0) / (2.0 * Kinc)) - 1.0;
-      n_eq[o] = n;
-    }
-    if (n_pol) n_pol[o] = (w * tj) / (4.0 * K);
+    if (turns_r) turns_r[o] = static_cast<int8_t>(radial_turns_at(rt, tj));
   }
 }
 
@@ -418,9 +422,21 @@ WsLayout ws_layout(int64_t npix, int64_t ngeo) {
   return L;
 }
 
+// The kernels always sample t, phi and sigma (as the reference's
+// raytrace_ana does, kerr_raytracing_ana.py:97): the theta pass parks G_phi,
+// G_t in the t and phi outputs and the plan holds the phi/t constants.  A
+// request without KGEO_DO_PHI_T is refused rather than run on kernel code
+// that no test has run.
+int require_phi_t(const kgeo_ray_params* p) {
+  if (!(p->flags & KGEO_DO_PHI_T))
+    return fail(KGEO_E_ARG, "KGEO_DO_PHI_T is required (t, phi and sigma are always sampled)%s%lld");
+  return KGEO_OK;
+}
+
 int validate(const kgeo_ray_params* p, const double* alpha, const double* beta, void* ws, size_t wsb,
              int64_t ngeo) {
   if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
+  if (require_phi_t(p)) return KGEO_E_ARG;
   if (p->npix < 0) return fail(KGEO_E_ARG, "npix < 0%s (%lld)", "", p->npix);
   if (p->npix > 0 && (!alpha || !beta)) return fail(KGEO_E_ARG, "alpha/beta is NULL%s%lld");
   if (p->npix > 0 && !ws) return fail(KGEO_E_WORKSPACE, "workspace is NULL%s%lld");
@@ -490,8 +506,7 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
                double* th, double* ph, double* sig, uint32_t* status, void* ws, hipStream_t st) {
   DevParams d = make_dev(p, ngeo);
   if (d.npix == 0 || ngeo <= 0) return KGEO_OK;
-  if (d.do_phi_t && (!t || !ph))
-    return fail(KGEO_E_ARG, "t and ph outputs are required with KGEO_DO_PHI_T%s%lld");
+  if (!t || !ph) return fail(KGEO_E_ARG, "t and ph outputs are required%s%lld");
   const uint32_t* gflags = static_cast<const uint32_t*>(ws);
   const double* plan = reinterpret_cast<const double*>(static_cast<const char*>(ws) + kFlagBytes);
   const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
@@ -500,19 +515,11 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   SampleArgs a{plan, gflags, SampleOut{tau_in, tau, t, r, th, ph, sig, d.npix, ngeo}, status, chunks};
   // the linspace grid walks (heads, far, walk, fixup passes); caller-given
   // Mino times and grids of at most two samples are evaluated directly
-  if (!tau_in && ngeo > 2)
-    return d.do_phi_t ? run_walk<true>(d, a, ws, grid, st) : run_walk<false>(d, a, ws, grid, st);
-  if (d.do_phi_t) {
-    hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);
-    int rc = check_launch("kg_theta");
-    if (rc) return rc;
-    hipLaunchKernelGGL(kg_radial<true>, grid, dim3(kBlock), 0, st, d, a);
-  } else {
-    hipLaunchKernelGGL(kg_theta<false>, grid, dim3(kBlock), 0, st, d, a);
-    int rc = check_launch("kg_theta");
-    if (rc) return rc;
-    hipLaunchKernelGGL(kg_radial<false>, grid, dim3(kBlock), 0, st, d, a);
-  }
+  if (!tau_in && ngeo > 2) return run_walk<true>(d, a, ws, grid, st);
+  hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);
+  int rc = check_launch("kg_theta");
+  if (rc) return rc;
+  hipLaunchKernelGGL(kg_radial<true>, grid, dim3(kBlock), 0, st, d, a);
   return check_launch("kg_radial");
 }
 
@@ -561,6 +568,7 @@ int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, do
                 double* th, double* ph, double* sig, uint32_t* status, void* workspace, size_t workspace_bytes,
                 void* stream) {
   if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
+  if (require_phi_t(p)) return KGEO_E_ARG;
   if (p->npix < 0 || ntau < 0) return fail(KGEO_E_ARG, "negative size%s%lld");
   if (p->npix > 0 && !workspace) return fail(KGEO_E_WORKSPACE, "workspace is NULL%s%lld");
   const size_t need = kgeo_workspace_bytes(p->npix, tau_in ? 0 : ntau);
@@ -606,6 +614,22 @@ int kgeo_crossings(const kgeo_ray_params* p, const double* alpha, const double* 
   hipLaunchKernelGGL(kg_counts, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), d, alpha, beta, tau,
                      ntau, n_equatorial, n_poloidal);
   return check_launch("kg_counts");
+}
+
+int kgeo_turnings(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau,
+                  int64_t ntau, double* n_angular, int32_t* turns_theta, int8_t* turns_r, void* stream) {
+  if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
+  if (p->npix < 0 || ntau < 0) return fail(KGEO_E_ARG, "negative size%s%lld");
+  // n_angular_turnings accepts th_o in [0, pi/2] only (kerr_raytracing_utils.py:690-691)
+  if ((n_angular || turns_theta) && !(p->th_o >= 0.0 && p->th_o <= kHalfPi))
+    return fail(KGEO_E_ARG, "th_o should be a float in range (0,pi/2]%s%lld");
+  if (p->npix == 0 || ntau == 0) return KGEO_OK;
+  if (!alpha || !beta || !tau) return fail(KGEO_E_ARG, "NULL array%s%lld");
+  DevParams d = make_dev(p, ntau);
+  const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(kg_turnings, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), d, alpha, beta, tau,
+                     ntau, n_angular, turns_theta, turns_r);
+  return check_launch("kg_turnings");
 }
 
 int kgeo_ellip_pi(const double* n, const double* phi, const double* m, double* out, int64_t count, void* stream) {

@@ -73,6 +73,19 @@ class Geodesics(object):
         from .counts import n_poloidal_orbits
         return n_poloidal_orbits(self.a, self.th_o, self.alpha, self.beta, self.tausteps)
 
+    def turning_counts(self):
+        """Turning points passed at every sample, (ngeo, npix) arrays computed on
+        the GPU: (theta, r).  theta = floor(n_angular_turnings) (int32,
+        kerr_raytracing_utils.py:686-792; needs th_o <= pi/2 as the reference
+        does); r = 1 once a ray outside the critical curve has passed its
+        turning point at half its total Mino time (int8)."""
+        from .counts import turnings_device
+        import torch
+        out = turnings_device(self.a, self.th_o, self.alpha, self.beta, self.tausteps, r_o=self.r_o,
+                              want=("turns_theta", "turns_r"))
+        torch.cuda.current_stream().synchronize()
+        return out["turns_theta"].cpu().numpy(), out["turns_r"].cpu().numpy()
+
     # labels computed with the rays (no reference attribute; rclass/thclass are
     # the reference's radial_roots / angular_turning classes)
     @property

@@ -98,10 +98,12 @@ def raytrace_ana_device(a, r_o, th_o, alpha, beta, ngeo, maxtaufrac=MAXTAUFRAC, 
     """Raytrace on the GPU and leave the results there.
 
     alpha, beta: 1-D tensors or arrays (float64).  Returns a RaytraceResult with
-    (ngeo, npix) tensors tau, t, r, th, ph, sig (those in ``want``), per-pixel
+    (ngeo, npix) tensors tau, t, r, th, ph, sig (those in ``want``; t and ph
+    are always produced, the theta pass parks its integrals in them), per-pixel
     rclass (int8), thclass (int8), nmax_eq (int32), tau_tot, status (int32).
     ``out`` may hold preallocated tensors of the same names.  Asynchronous on
-    torch's current stream.
+    torch's current stream.  do_phi_and_t is accepted and ignored, as in the
+    reference's raytrace_ana (kerr_raytracing_ana.py:97 always samples t and phi).
     """
     lib = _lib.load()
     dev = _device()
@@ -113,7 +115,7 @@ def raytrace_ana_device(a, r_o, th_o, alpha, beta, ngeo, maxtaufrac=MAXTAUFRAC, 
     out = dict(out or {})
     f64 = dict(device=dev, dtype=torch.float64)
     for k in ("tau", "t", "r", "th", "ph", "sig"):
-        if k in want and k not in out:
+        if (k in want or k in ("t", "ph")) and k not in out:
             out[k] = torch.empty((ngeo, npix), **f64)
     for k, dt in (("rclass", torch.int8), ("thclass", torch.int8), ("nmax_eq", torch.int32),
                   ("tau_tot", torch.float64), ("status", torch.int32)):
@@ -123,7 +125,7 @@ def raytrace_ana_device(a, r_o, th_o, alpha, beta, ngeo, maxtaufrac=MAXTAUFRAC, 
     ws = out.get("workspace")
     if ws is None or ws.numel() < wsb:
         ws = torch.empty(wsb + 256, device=dev, dtype=torch.uint8)
-    p = _lib.make_params(a, r_o, th_o, npix, ngeo, maxtaufrac, do_phi_and_t)
+    p = _lib.make_params(a, r_o, th_o, npix, ngeo, maxtaufrac)
     rc = lib.kgeo_raytrace_ana(p, _ptr(al), _ptr(be), _ptr(out.get("tau")), _ptr(out.get("t")), _ptr(out.get("r")),
                                _ptr(out.get("th")), _ptr(out.get("ph")), _ptr(out.get("sig")),
                                _ptr(out["rclass"]), _ptr(out["thclass"]), _ptr(out["nmax_eq"]),
@@ -136,7 +138,13 @@ def raytrace_ana_device(a, r_o, th_o, alpha, beta, ngeo, maxtaufrac=MAXTAUFRAC, 
 
 def coords_at_tau_device(a, r_o, th_o, alpha, beta, tau, do_phi_and_t=True):
     """Coordinates at caller-given Mino times tau (ntau, npix) or (npix,) on the GPU.
-    Returns (sig, [t, r, th, ph]) as (ntau, npix) tensors."""
+    Returns (sig, [t, r, th, ph]) as (ntau, npix) tensors.
+
+    With do_phi_and_t=False the reference's th_integrate / r_integrate leave
+    G_phi, G_t, I_+-, I_1, I_2 at zero (kerr_raytracing_ana.py:212-214,
+    394-398), so coords_at_tau returns sigma = 0, phi = 0 and t = 4 I_0 =
+    -4 tau (:186-188, :626-631); those values are returned here, with r and
+    theta from the kernels (which always evaluate the full integrals)."""
     lib = _lib.load()
     dev = _device()
     al = _as_device_f64(alpha, dev).reshape(-1)
@@ -153,10 +161,14 @@ def coords_at_tau_device(a, r_o, th_o, alpha, beta, tau, do_phi_and_t=True):
     status = torch.empty(npix, device=dev, dtype=torch.int32)
     wsb = _lib.workspace_bytes(npix)
     ws = torch.empty(wsb + 256, device=dev, dtype=torch.uint8)
-    p = _lib.make_params(a, r_o, th_o, npix, ntau, MAXTAUFRAC, do_phi_and_t)
+    p = _lib.make_params(a, r_o, th_o, npix, ntau, MAXTAUFRAC)
     rc = lib.kgeo_coords_at_tau(p, _ptr(al), _ptr(be), _ptr(ta), ntau, _ptr(o["t"]), _ptr(o["r"]), _ptr(o["th"]),
                                 _ptr(o["ph"]), _ptr(o["sig"]), _ptr(status), _ptr(ws), wsb, _stream_handle())
     _lib.check(rc, "kgeo_coords_at_tau")
+    if not do_phi_and_t:
+        o["sig"].zero_()
+        o["ph"].zero_()
+        torch.mul(ta, -4.0, out=o["t"])
     return o["sig"], [o["t"], o["r"], o["th"], o["ph"]]
 
 
@@ -206,6 +218,6 @@ def coords_at_tau(a, observer_coords, image_coords, tau, do_phi_and_t=True):
     if not (tau.shape[1] == len(alpha)):
         raise Exception("tau has incompatible shape in coords_at_tau!")
     _minspin_check(a, th_o, alpha, beta)
-    sig, (t, r, th, ph) = coords_at_tau_device(a, r_o, th_o, alpha, beta, tau, do_phi_and_t=True)
+    sig, (t, r, th, ph) = coords_at_tau_device(a, r_o, th_o, alpha, beta, tau, do_phi_and_t=do_phi_and_t)
     torch.cuda.current_stream().synchronize()
     return sig.cpu().numpy(), [t.cpu().numpy(), r.cpu().numpy(), th.cpu().numpy(), ph.cpu().numpy()]

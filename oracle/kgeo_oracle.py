@@ -754,6 +754,78 @@ def n_poloidal_orbits(a, th_o, alpha, beta, tau):
     return out[0] if one else out
 
 
+def n_angular_turnings(a, th_o, alpha, beta, tau):
+    """Fractional number of angular turning points (kerr_raytracing_utils.py:686-792).
+
+    Ordinary rays (eta > 0): (tau sqrt(-a^2 u_-) -+ F0) / (2K) + 1/2, minus F0
+    for beta <= 0 (:744-749).  Rays with eta <= 0 take the reference's
+    vortical formula (:753-786) as written, including its "norbit > 0.5 ->
+    + 0.5" adjustment for beta <= 0 (:780-781), which the reference itself
+    flags as unverified; eta == 0 rays come out NaN there (sqrt of a^2 u_- < 0).
+    Like the reference, th_o must lie in [0, pi/2] (:690-691).
+    """
+    if not (isinstance(a, float) and (0 <= np.abs(a) < 1)):
+        raise Exception("|a| should be a float in range [0,1)")
+    if not (isinstance(th_o, float) and (0 <= th_o <= np.pi / 2.)):
+        raise Exception("th_o should be a float in range (0,pi/2]")
+    alpha, beta = _as1d(alpha), _as1d(beta)
+    one = len(tau.shape) == 1
+    if one:
+        tau = tau.reshape(1, tau.shape[0])
+    lam, eta = conserved(a, th_o, alpha, beta)
+    out = np.empty(tau.shape)
+    vort = eta <= 0
+    out[:, vort] = 0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        if np.any(~vort):
+            bet = beta[~vort]
+            tr = tau[:, ~vort]
+            up, um, ratio, a2um = uplus_uminus(a, th_o, lam[~vort], eta[~vort])
+            F0 = sp.ellipkinc(np.arcsin(np.cos(th_o) / np.sqrt(up)), ratio)
+            K = sp.ellipkinc(0.5 * np.pi, ratio)
+            w = np.sqrt(-a2um)
+            res = np.empty(tr.shape)
+            bm = bet <= 0
+            if np.any(bm):
+                res[:, bm] = ((tr * w - F0) / (2 * K))[:, bm] + 0.5
+            if np.any(~bm):
+                res[:, ~bm] = ((tr * w + F0) / (2 * K))[:, ~bm] + 0.5
+            out[:, ~vort] = res
+        if np.any(vort):
+            bet = beta[vort]
+            tv = tau[:, vort]
+            up, um, ratio, a2um = uplus_uminus(a, th_o, lam[vort], eta[vort])
+            Nu = np.arcsin(np.sqrt((np.cos(th_o) ** 2 - um) / (up - um)))
+            F0 = sp.ellipkinc(Nu, 1 - ratio)
+            K = sp.ellipkinc(0.5 * np.pi, 1 - ratio)
+            w = np.sqrt(a2um)
+            res = np.empty(tv.shape)
+            bm = bet <= 0
+            if np.any(bm):
+                norbit = ((tv * w - F0) / (2 * K))[:, bm] + 0.5
+                norbit[norbit > 0.5] += 0.5
+                res[:, bm] = norbit
+            if np.any(~bm):
+                res[:, ~bm] = ((tv * w + F0) / (2 * K))[:, ~bm] + 0.5
+            out[:, vort] = res
+    return out[0] if one else out
+
+
+def radial_turnings(a, th_o, alpha, beta, tau, tau_tot):
+    """Radial turning points passed by Mino time tau: 1 where the ray lies
+    outside the critical curve (kerr_raytracing_utils.py:794-826) and tau is
+    past half its total Mino time, else 0.  A ray outside the critical curve
+    reaches its one radial turning point r_4 at tau_tot / 2
+    (kerr_raytracing_utils.py:518-546: tau_tot = 2 I_r(r_o)); the reference
+    tests the same condition as `not (I_r <= I_max / 2)` when it picks the
+    radial momentum sign (equatorial_images.py:266-288).  int8, tau's shape."""
+    alpha, beta = _as1d(alpha), _as1d(beta)
+    out_crit = is_outside_crit(a, th_o, alpha, beta).astype(bool)
+    with np.errstate(invalid="ignore"):
+        passed = ~(tau <= 0.5 * tau_tot)
+    return (passed & out_crit).astype(np.int8)
+
+
 # ---------------------------------------------------------------------------
 # equatorial emission  (equatorial_lensing.py:85-196, equatorial_images.py,
 # velocities.py:229-281, kerr_raytracing_utils.py:794-826)

@@ -157,6 +157,21 @@ class Host:
         return dict(rclass=rc, thclass=tc, rcase=cs, status=st, tau_tot=tt, roots=r8.reshape(n, 8),
                     upum=uu.reshape(n, 2))
 
+    def counts(self, a, r_o, th_o, alpha, beta, tau):
+        """The count kernels' formulas (kg_counts.hpp) on the CPU at Mino times
+        tau (ntau, npix): n_equatorial, n_poloidal, n_angular, turns_theta, turns_r."""
+        alpha = np.ascontiguousarray(alpha, float)
+        beta = np.ascontiguousarray(beta, float)
+        tau = np.ascontiguousarray(tau, float).reshape(-1, len(alpha))
+        ntau, n = tau.shape
+        out = dict(n_equatorial=np.empty((ntau, n)), n_poloidal=np.empty((ntau, n)), n_angular=np.empty((ntau, n)),
+                   turns_theta=np.empty((ntau, n), np.int32), turns_r=np.empty((ntau, n), np.int8))
+        self.lib.kgh_counts(self._p(self.scalars(a, r_o, th_o)), self._p(alpha), self._p(beta), ctypes.c_int64(n),
+                            self._p(tau), ctypes.c_int64(ntau), self._p(out["n_equatorial"]),
+                            self._p(out["n_poloidal"]), self._p(out["n_angular"]),
+                            self._p(out["turns_theta"], ctypes.c_int32), self._p(out["turns_r"], ctypes.c_int8))
+        return out
+
     def raytrace(self, a, r_o, th_o, alpha, beta, ngeo, tau_in=None):
         n = len(alpha)
         outs = {k: np.empty((ngeo, n)) for k in ("tau", "t", "r", "th", "ph", "sig")}

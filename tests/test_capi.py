@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_load_and_version():
     lib = _lib.load()
-    assert lib.kgeo_abi_version() == 2
+    assert lib.kgeo_abi_version() == 3
     assert _lib.workspace_bytes(1000) == 256 + 1000 * _plan_slots() * 8
 
 
@@ -67,7 +67,7 @@ def test_fill_params_matches_numpy_expressions():
         assert abs(getattr(p, f) - getattr(q, f)) <= 2e-16 * max(1, abs(getattr(q, f))), f
 
 
-@pytest.mark.parametrize("bad", ["null_params", "neg_npix", "no_workspace", "small_workspace", "ngeo0"])
+@pytest.mark.parametrize("bad", ["null_params", "neg_npix", "no_workspace", "small_workspace", "ngeo0", "no_phi_t"])
 def test_argument_errors(bad):
     lib = _lib.load()
     p = _lib.make_params(0.5, 1e4, 0.5, 10, 10, 1.0)
@@ -86,10 +86,53 @@ def test_argument_errors(bad):
         wsb = 16
     elif bad == "ngeo0":
         p.ngeo = 0
+    elif bad == "no_phi_t":
+        p.flags = 0
     rc = lib.kgeo_raytrace_ana(pp, fake, fake, None, None, None, None, None, None, None, None, None, None, None,
                                ws, wsb, None)
     assert rc < 0
     assert lib.kgeo_last_error()
+
+
+def test_every_raytrace_entry_requires_phi_t():
+    """t, phi and sigma are always sampled (the reference's raytrace_ana does,
+    kerr_raytracing_ana.py:97): requests without KGEO_DO_PHI_T are refused
+    before any device work, by every entry point that samples rays."""
+    lib = _lib.load()
+    p = _lib.make_params(0.5, 1e4, 0.5, 10, 10, 1.0, do_phi_t=False)
+    buf = (ctypes.c_char * 8192)()
+    fake = ctypes.c_void_p((ctypes.addressof(buf) + 255) // 256 * 256)
+    rcs = [
+        lib.kgeo_plan(ctypes.byref(p), fake, fake, None, None, None, None, None, fake, 1 << 40, None),
+        lib.kgeo_sample(ctypes.byref(p), None, 10, None, fake, fake, fake, fake, None, None, fake, 1 << 40, None),
+        lib.kgeo_coords_at_tau(ctypes.byref(p), fake, fake, fake, 10, fake, fake, fake, fake, None, None, fake,
+                               1 << 40, None),
+    ]
+    assert all(rc == -1 for rc in rcs), rcs
+    assert b"KGEO_DO_PHI_T" in lib.kgeo_last_error()
+    imgs = (_lib.RayParams * 2)(_lib.make_params(0.5, 1e4, 0.5, 64, 10, 1.0),
+                                _lib.make_params(0.5, 1e4, 0.5, 64, 10, 1.0, do_phi_t=False))
+    assert lib.kgeo_batch_workspace_bytes(imgs, 2) == 0
+    h = ctypes.c_void_p()
+    rc = lib.kgeo_batch_plan(imgs, 2, None, fake, fake, fake, fake, fake, fake, None, None, None, None, None,
+                             fake, 1 << 40, fake, ctypes.byref(h))
+    assert rc == -1 and b"KGEO_DO_PHI_T" in lib.kgeo_batch_last_error()
+
+
+def test_turnings_argument_checks():
+    """n_angular_turnings accepts th_o in [0, pi/2] only (kerr_raytracing_utils.py:690-691);
+    the radial count alone has no such limit (checked before any launch)."""
+    lib = _lib.load()
+    buf = (ctypes.c_char * 64)()
+    fake = ctypes.c_void_p(ctypes.addressof(buf))
+    p = _lib.make_params(0.5, 1e4, 2.0, 4, 1, 1.0)
+    assert lib.kgeo_turnings(ctypes.byref(p), fake, fake, fake, 1, fake, None, None, None) == -1
+    assert b"(0,pi/2]" in lib.kgeo_last_error()
+    assert lib.kgeo_turnings(ctypes.byref(p), fake, fake, fake, 1, None, fake, None, None) == -1
+    # no pixels: accepted without touching the device
+    p0 = _lib.make_params(0.5, 1e4, 2.0, 0, 1, 1.0)
+    assert lib.kgeo_turnings(ctypes.byref(p0), None, None, None, 1, None, None, fake, None) == 0
+    assert lib.kgeo_turnings(ctypes.byref(p), None, None, None, -1, None, None, None, None) == -1
 
 
 def test_product_path_requires_gpu():

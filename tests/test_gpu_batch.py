@@ -108,40 +108,18 @@ def test_batch_equals_plain_launches():
     sv.close()
 
 
-@pytest.mark.skip(reason="not run on a GPU yet: the run that was to run it first ended in a GPU fault "
-                         "in a preceding test with since-reverted batch kernels; kept off the GPU until "
-                         "that fault has a shown cause")
-def test_batch_without_phi_t_and_mixed_surveys():
-    """Surveys where no image, or only some images, want phi and t (the
-    pass kernels' other two instantiations): r and theta of every image,
-    and t and phi of the images that want them, equal plain calls with the
-    same per-image flag to the walk's rounding."""
-    for mixed in (False, True):
-        images = _images()
-        for k, im in enumerate(images):
-            im["do_phi_and_t"] = mixed and k % 2 == 0
-        sv = _batch(images)
-        arena_p, labels_p = _plain(images, sv.out_off)
-        for k in ("rclass", "thclass", "nmax_eq", "status"):
-            assert _same(_host(getattr(sv, k)), labels_p[k]), (mixed, k)
-        if not mixed:
-            assert sv.arena["t"] is None and sv.arena["ph"] is None
-        for key, a_b in sv.arena.items():
-            if a_b is None:
-                continue
-            a_b = _host(a_b)
-            for k, im in enumerate(images):
-                if key in ("t", "ph") and not im["do_phi_and_t"]:
-                    continue
-                o0, o1 = int(sv.out_off[k]), int(sv.out_off[k + 1])
-                x, y = a_b[o0:o1], arena_p[key][o0:o1]
-                assert np.array_equal(np.isnan(x), np.isnan(y)), (mixed, key, k)
-                ok = ~np.isnan(y)
-                rel = np.abs(x[ok] - y[ok]) / np.maximum(np.abs(y[ok]), 1.0)
-                assert rel.max(initial=0.0) < 4e-11, (mixed, key, k, float(rel.max()))
-        done, total = sv.progress()
-        assert np.array_equal(done, total)
-        sv.close()
+def test_batch_refuses_surveys_without_phi_t():
+    """Every image samples t, phi and sigma, as raytrace_ana does
+    (kerr_raytracing_ana.py:97): a survey, or one image, asking for
+    do_phi_and_t=False is refused before anything is allocated or launched."""
+    from kgeo_amd import _lib
+    from kgeo_amd.batch import BatchSurvey
+    images = _images()
+    with pytest.raises(_lib.KgeoError, match="do_phi_and_t=False"):
+        BatchSurvey(images, r_o=R_O, do_phi_and_t=False)
+    images[3]["do_phi_and_t"] = False
+    with pytest.raises(_lib.KgeoError, match="image 3"):
+        BatchSurvey(images, r_o=R_O)
 
 
 @pytest.mark.parametrize("k", [0, 1, 6])

@@ -1,9 +1,16 @@
 """GPU parity of the HIP raytracer against the oracle, through the C-ABI.
 
 Bar: integer labels (rclass, thclass, nmax_eq, equatorial crossing counts)
-bit-exact; r, theta, phi, t, sigma within 1e-10 (metrics in
-tests/kgeo_testlib.errors) on every sample the oracle itself resolves at
-that level (conditioning_mask), with max and p99 over all samples printed.
+bit-exact; r, theta, phi, t, sigma (metrics in tests/kgeo_testlib.errors):
+  * within 1e-10 on every sample the oracle itself resolves at that level
+    (conditioning_mask);
+  * p99 over all samples within 1e-10;
+  * over the samples with tau <= (1 - 1e-6) tau_tot (all but the horizon
+    end of infalling rays) within 1e-9, with the count above 1e-10 printed
+    (at 512^2 x 500 two phi samples of 2,048,000: a near-polar ray and a
+    sample whose oracle value itself moves by 6e-11 when tau moves by 1e-14);
+  * within 1e-3 on the ill-conditioned samples (the horizon end, where
+    r - r_+ is known to ~1e-6 relative and phi, t hold log(r - r_+)).
 """
 
 import json
@@ -30,25 +37,35 @@ def _gpu(a, r_o, th, A, B, G, **kw):
     return out
 
 
-def _check(got, ref, a, r_o, th, A, B, label):
+def _check(got, ref, a, r_o, th, A, B, label, gate_p99=False):
+    """gate_p99: assert p99 only where the ray ends are under 0.5 % of the
+    samples (short grids, where the last sample of each infalling ray is a
+    large share of all samples); the C1 and C2 workloads assert it always."""
     assert np.array_equal(got["rclass"].astype(int), ref["rclass"]), label
     assert np.array_equal(got["thclass"].astype(int), ref["thclass"]), label
     nm = O.n_equatorial_crossings(a, th, A, B, ref["tau"][-1])
     assert np.array_equal(got["nmax_eq"].astype(float), nm), label
     assert np.max(np.abs(got["tau"] - ref["tau"]) / np.maximum(ref["tau"], 1e-300)) < 1e-12
     ill = conditioning_mask(a, r_o, th, A, B, ref["tau"], ref)
+    early = ref["tau"] <= (1.0 - 1.0e-6) * ref["tau"][-1][None, :]
     rep = {}
     for k in ("r", "th", "ph", "t", "sig"):
         s = summarize(got[k], ref[k], k, ill)
+        e = np.nan_to_num(errors(got[k], ref[k], k))
+        s["max_early"] = float(np.max(np.where(early, e, 0.0)))
+        s["n_early_above_1e-10"] = int(np.sum(early & (e > 1e-10)))
         rep[k] = s
         assert s["nan_mismatch"] == 0, (label, k, s)
         assert s["max_well"] < 1e-10, (label, k, s)
-        if ill.mean() < 0.005:   # p99 over all samples is meaningful when the ray ends are <1% of them
+        if not gate_p99 or ill.mean() < 0.005:
             assert s["p99"] < 1e-10, (label, k, s)
+        assert s["max_early"] < 1e-9, (label, k, s)
+        assert s["max_ill"] < 1e-3, (label, k, s)
     print(f"\n[{label}] ill-conditioned samples: {int(ill.sum())} of {ill.size}")
     for k, s in rep.items():
         print(f"  {k:3s} max {s['max']:.2e}  p99 {s['p99']:.2e}  max(well-conditioned) {s['max_well']:.2e}"
-              f"  max(ill) {s['max_ill']:.2e}")
+              f"  max(ill) {s['max_ill']:.2e}  max(tau<=(1-1e-6)tau_tot) {s['max_early']:.2e}"
+              f" ({s['n_early_above_1e-10']} above 1e-10)")
     return rep
 
 
@@ -163,7 +180,7 @@ def test_other_regimes(case):
     A, B = grid(W, n)
     ref = O.raytrace_ana(a, 1e4, th, A, B, G)
     got = _gpu(a, 1e4, th, A, B, G)
-    _check(got, ref, a, 1e4, th, A, B, f"a={a} th={th_deg} n={n} G={G}")
+    _check(got, ref, a, 1e4, th, A, B, f"a={a} th={th_deg} n={n} G={G}", gate_p99=True)
 
 
 def test_single_pixel_and_ngeo_one():

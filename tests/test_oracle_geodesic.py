@@ -46,7 +46,7 @@ def test_initial_conditions():
         assert np.max(np.abs(out[k][0]) / scale) < 1e-10, k
 
 
-@pytest.mark.parametrize("a,th_deg", [(0.94, 17.0), (0.5, 60.0), (-0.7, 80.0)])
+@pytest.mark.parametrize("a,th_deg", [(0.94, 17.0), (0.5, 60.0), (-0.7, 80.0), (0.9, 140.0)])
 def test_mino_time_geodesic_equations(a, th_deg):
     th = np.deg2rad(th_deg)
     A, B = _sample_pixels(a, th)
