@@ -211,6 +211,24 @@ int kgeo_equatorial_image_vjp(const kgeo_ray_params* p, const double* alpha, con
 const char* kgeo_equatorial_last_error(void);
 
 /*
+ * No counterpart in the reference.  kgeo_equatorial_image_vjp without a
+ * workspace, for callers that cannot allocate device memory (code under HIP
+ * graph capture, e.g. a captured optimiser step over (a, th_o, p); tight
+ * loops; harnesses that forbid hipMalloc).  It calls no hipMalloc / hipFree:
+ * the per-block partial sums live in a static device array inside the
+ * library, one copy per GPU (768 KiB, held until the process ends), whose
+ * address is looked up once per device on the first call there.  Results
+ * are bit-identical to kgeo_equatorial_image_vjp.  npix is limited to
+ * KGEO_VJP_NOALLOC_MAX_NPIX (KGEO_E_ARG beyond).  Every call on one GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (two
+ * streams, two threads) race and must be serialised by the caller.
+ */
+#define KGEO_VJP_NOALLOC_MAX_NPIX 8388608
+int kgeo_equatorial_image_vjp_noalloc(const kgeo_ray_params* p, const double* alpha, const double* beta,
+                                      int mbar_max, double emis_p, const double* grad_image, double* jac,
+                                      double* grad3, void* stream);
+
+/*
  * No counterpart in the reference (its callers loop over raytrace_ana, one
  * image per call).  Batched survey: nimg images, each with its own
  * kgeo_ray_params (a, r_o, th_o, npix, ngeo, flags), traced by one

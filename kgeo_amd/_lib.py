@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libkgeo_hip.so")
 
 KGEO_DO_PHI_T = 1
 
+VJP_NOALLOC_MAX_NPIX = 8388608   # KGEO_VJP_NOALLOC_MAX_NPIX
+
 ST_NONFINITE = 1
 ST_DEGENERATE_ROOTS = 2
 ST_MINSPIN_VORTICAL = 4
@@ -76,6 +78,9 @@ _PROTOS = {
                                                  ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 4
                                   + [ctypes.c_size_t, ctypes.c_void_p]),
     "kgeo_equatorial_last_error": (ctypes.c_char_p, []),
+    "kgeo_equatorial_image_vjp_noalloc": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p,
+                                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+                                          + [ctypes.c_void_p] * 4),
     "kgeo_crossings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
                        + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "kgeo_turnings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <mutex>
 
 #include "../../include/kgeo.h"
 #include "kg_dual.hpp"
@@ -161,6 +162,68 @@ int eq_launch(const char* what) {
   return KGEO_OK;
 }
 
+int vjp_check(const kgeo_ray_params* p, const double* alpha, const double* beta, int mbar_max, double* grad3) {
+  if (!p) return eq_fail(KGEO_E_ARG, "params is NULL");
+  if (p->npix < 0) return eq_fail(KGEO_E_ARG, "npix < 0");
+  if (mbar_max < 0 || mbar_max >= kMaxSubrings) return eq_fail(KGEO_E_ARG, "mbar_max outside [0, 3]");
+  if (!grad3) return eq_fail(KGEO_E_ARG, "grad3 is NULL");
+  if (p->npix > 0 && (!alpha || !beta)) return eq_fail(KGEO_E_ARG, "alpha/beta is NULL");
+  return KGEO_OK;
+}
+
+// The tangent kernel writes one partial triple per block into `partials`
+// (every block writes its own, so nothing needs clearing first); the reduce
+// kernel sums them in a fixed order into grad3.
+int run_vjp(const kgeo_ray_params* p, const double* alpha, const double* beta, int mbar_max, double emis_p,
+            const double* grad_image, double* jac, double* grad3, double* partials, hipStream_t st) {
+  TanParams tp{p->a, p->th_o, p->r_o, emis_p, p->npix, mbar_max};
+  const int64_t nb = (p->npix + kBlock - 1) / kBlock;
+  if (nb > 0) {
+    hipLaunchKernelGGL(kg_eq_tangent, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, st, tp, alpha, beta,
+                       grad_image, jac, partials);
+    int rc = eq_launch("kg_eq_tangent");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(kg_eq_reduce, dim3(1), dim3(kBlock), 0, st, partials, nb, grad3);
+  return eq_launch("kg_eq_reduce");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of kgeo_equatorial_image_vjp_noalloc: the block partials live in
+// this file-scope device array instead of a caller workspace.  Each GPU the
+// library runs on holds its own copy (768 KiB) until the process ends.  Host
+// code never uses the array's name as an address: scratch_ptr() asks the
+// runtime for the device address once per device (with that device current)
+// and keeps it.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kScratchBlocks = 32768;  // partial triples: 32768 blocks of 256 pixels
+static_assert(kScratchBlocks * kBlock == KGEO_VJP_NOALLOC_MAX_NPIX, "scratch size and the header's limit differ");
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[3 * kScratchBlocks];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_ptr(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return eq_fail(KGEO_E_HIP, "hipGetDevice failed");
+  if (dev < 0 || dev >= kMaxDevices) return eq_fail(KGEO_E_ARG, "device ordinal outside [0, 63]");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+      return eq_fail(KGEO_E_HIP, "hipGetSymbolAddress(scratch) failed");
+    g_scratch[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return KGEO_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -202,25 +265,27 @@ int kgeo_equatorial_image(const kgeo_ray_params* p, const double* alpha, const d
 int kgeo_equatorial_image_vjp(const kgeo_ray_params* p, const double* alpha, const double* beta, int mbar_max,
                               double emis_p, const double* grad_image, double* jac, double* grad3, void* workspace,
                               size_t workspace_bytes, void* stream) {
-  if (!p) return eq_fail(KGEO_E_ARG, "params is NULL");
-  if (p->npix < 0) return eq_fail(KGEO_E_ARG, "npix < 0");
-  if (mbar_max < 0 || mbar_max >= kMaxSubrings) return eq_fail(KGEO_E_ARG, "mbar_max outside [0, 3]");
-  if (!grad3) return eq_fail(KGEO_E_ARG, "grad3 is NULL");
-  if (p->npix > 0 && (!alpha || !beta)) return eq_fail(KGEO_E_ARG, "alpha/beta is NULL");
+  int rc = vjp_check(p, alpha, beta, mbar_max, grad3);
+  if (rc) return rc;
   if (!workspace || workspace_bytes < kgeo_equatorial_vjp_workspace_bytes(p->npix))
     return eq_fail(KGEO_E_WORKSPACE, "workspace too small");
-  TanParams tp{p->a, p->th_o, p->r_o, emis_p, p->npix, mbar_max};
-  const int64_t nb = (p->npix + kBlock - 1) / kBlock;
-  double* partials = static_cast<double*>(workspace);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (nb > 0) {
-    hipLaunchKernelGGL(kg_eq_tangent, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, st, tp, alpha, beta,
-                       grad_image, jac, partials);
-    int rc = eq_launch("kg_eq_tangent");
-    if (rc) return rc;
-  }
-  hipLaunchKernelGGL(kg_eq_reduce, dim3(1), dim3(kBlock), 0, st, partials, nb, grad3);
-  return eq_launch("kg_eq_reduce");
+  return run_vjp(p, alpha, beta, mbar_max, emis_p, grad_image, jac, grad3, static_cast<double*>(workspace),
+                 static_cast<hipStream_t>(stream));
+}
+
+int kgeo_equatorial_image_vjp_noalloc(const kgeo_ray_params* p, const double* alpha, const double* beta,
+                                      int mbar_max, double emis_p, const double* grad_image, double* jac,
+                                      double* grad3, void* stream) {
+  int rc = vjp_check(p, alpha, beta, mbar_max, grad3);
+  if (rc) return rc;
+  if (p->npix > KGEO_VJP_NOALLOC_MAX_NPIX)
+    return eq_fail(KGEO_E_ARG, "npix exceeds the allocation-free scratch (8,388,608 pixels); "
+                               "use kgeo_equatorial_image_vjp with a workspace");
+  double* partials = nullptr;
+  rc = scratch_ptr(&partials);
+  if (rc) return rc;
+  return run_vjp(p, alpha, beta, mbar_max, emis_p, grad_image, jac, grad3, partials,
+                 static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"
