@@ -211,6 +211,37 @@ int kgeo_equatorial_image_vjp(const kgeo_ray_params* p, const double* alpha, con
 const char* kgeo_equatorial_last_error(void);
 
 /*
+ * Replaces rho_of_req / rho_of_req_single (equatorial_lensing.py:226-262)
+ * and the root finds of generate_library (:351-416) for many curves in one
+ * launch.  Curve k is the lensed image of the equatorial circle r = req[k]
+ * in subring mbar[k] for the observer curves[k] (a, th_o and the derived
+ * scalars as for kgeo_raytrace_ana; the reference places the observer at
+ * r_o = infinity, :19, :202).  For each screen angle varphi[i]:
+ *   rho[k][i]    the screen radius at which r_equatorial2 (:198-216: r_s of
+ *                the mbar-th crossing; 1e50 for rays outside the critical
+ *                curve without that crossing, 1 for rays inside it) equals
+ *                req[k], by Brent's method over the reference's bracket
+ *                (mbar 0: [0, req + 5]; 1: [0, 100]; else [1, 10]) with
+ *                brentq's tolerances (xtol 2e-12, rtol 4 eps, 100 iterations)
+ *   status[k][i] optional int32: 0 found; 1 no sign change over the bracket
+ *                (brentq raises ValueError there); 2 not converged
+ *                (RuntimeError); rho is NaN unless 0
+ * All arrays are device arrays, curves included ([ncurves] kgeo_ray_params).
+ */
+int kgeo_rho_of_req(const kgeo_ray_params* curves, const double* req, const int32_t* mbar, int64_t ncurves,
+                    const double* varphi, int64_t nphi, double* rho, int32_t* status, void* stream);
+
+/*
+ * Replaces the closed form of critical_curve (equatorial_lensing.py:264-293)
+ * at photon-orbit radii r [n]: alpha [n], beta [n] >= 0 of the bound orbit at
+ * each radius for p->a, p->th_o (NaN where it is not seen from th_o).  The
+ * caller builds r = linspace(r_-, r_+, n) and drops the NaN points and
+ * mirrors beta as the reference does (kgeo_amd.lensing.critical_curve).
+ */
+int kgeo_critical_curve(const kgeo_ray_params* p, const double* r, int64_t n, double* alpha, double* beta,
+                        void* stream);
+
+/*
  * No counterpart in the reference.  kgeo_equatorial_image_vjp without a
  * workspace, for callers that cannot allocate device memory (code under HIP
  * graph capture, e.g. a captured optimiser step over (a, th_o, p); tight

@@ -78,6 +78,10 @@ _PROTOS = {
                                                  ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 4
                                   + [ctypes.c_size_t, ctypes.c_void_p]),
     "kgeo_equatorial_last_error": (ctypes.c_char_p, []),
+    "kgeo_rho_of_req": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64]
+                        + [ctypes.c_void_p] * 3),
+    "kgeo_critical_curve": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_int64]
+                            + [ctypes.c_void_p] * 3),
     "kgeo_equatorial_image_vjp_noalloc": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p,
                                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
                                           + [ctypes.c_void_p] * 4),

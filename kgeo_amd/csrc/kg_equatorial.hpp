@@ -89,41 +89,37 @@ KG_INLINE T redshift(const T& a, const T& r, const T& lam, const T& eta, double 
   return 1.0 / (u0 - lam * u3 - th_term - (kr_sign * u1) * ksqrt(R) / Delta);
 }
 
-// One pixel of the equatorial image for subrings 0..mbar_max.
+// The subring-independent part of r_equatorial for one pixel
+// (equatorial_lensing.py:110-157): lambda, eta, K, F0, the roots, I_max,
+// N_max and the radial constants; and is_outside_crit
+// (kerr_raytracing_utils.py:821-823).  eta <= 0 (vortical) leaves the rest
+// unset (equatorial_lensing.py:119-123).
+template <class T> struct EqRay {
+  T lam, eta, K, Fo, w, Im;
+  double N;
+  bool vort, outside;
+  RadialConsts<T> rc;
+  JacobiLadder<T> lad;
+};
+
 template <class T>
-KG_INLINE void equatorial_pixel(const RayScalars<T>& c, const T& alpha, const T& beta, const T& p_emis,
-                                const T& ri, int mbar_max, EqPixel<T>& o) {
-  o.image = T(0.0);
-  o.zero_mask = 0;
-  for (int m = 0; m < kMaxSubrings; ++m) {
-    o.I[m] = T(0.0);
-    o.g[m] = T(0.0);
-    o.r_s[m] = T(0.0);
-    o.Ir[m] = T(0.0);
-  }
+KG_INLINE void eq_ray(const RayScalars<T>& c, const T& alpha, const T& beta, EqRay<T>& e) {
   PixelSetup<T> px;
   px.status = 0;
   {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
-    px.lam = -alpha * c.sin_th;
-    px.eta = (alpha * alpha - c.a2) * c.cos2_th + beta * beta;
+    e.lam = -alpha * c.sin_th;
+    e.eta = (alpha * alpha - c.a2) * c.cos2_th + beta * beta;
   }
-  const T& lam = px.lam;
-  const T& eta = px.eta;
-  if (val(eta) <= 0.0) {  // vortical sentinels (equatorial_lensing.py:119-123)
-    for (int m = 0; m <= mbar_max; ++m) {
-      o.r_s[m] = T(-1.0);
-      o.Ir[m] = T(-1.0);
-      o.zero_mask |= 1 << m;
-    }
-    o.Imax = T(-1.0);
-    o.Nmax = -2;
-    return;
-  }
-  T up, um, ratio, a2um, Fo, K, w, Im;
-  double N;
+  const T& lam = e.lam;
+  const T& eta = e.eta;
+  e.vort = val(eta) <= 0.0;
+  e.outside = false;
+  e.N = -2.0;
+  if (e.vort) return;
+  T up, um, ratio, a2um;
   {
 #if defined(__clang__)
 #pragma clang fp contract(off)
@@ -136,56 +132,87 @@ KG_INLINE void equatorial_pixel(const RayScalars<T>& c, const T& alpha, const T&
       ratio = up / um;
       a2um = c.a2 * um;
     }
-    K = ellip_K(ratio);
-    Fo = legendre_F(legendre_base(amp_from_sin(c.cos_th / ksqrt(up)), ratio), K);
+    e.K = ellip_K(ratio);
+    e.Fo = legendre_F(legendre_base(amp_from_sin(c.cos_th / ksqrt(up)), ratio), e.K);
     radial_roots(c, lam, eta, px);
-    Im = mino_total(c, px);
-    w = ksqrt(-a2um);
-    const double x = val(Im) * val(w);
+    e.Im = mino_total(c, px);
+    e.w = ksqrt(-a2um);
+    const double x = val(e.Im) * val(e.w);
+    const double Fo = val(e.Fo), K = val(e.K);
     if (c.h > 0.0) {
-      N = (val(beta) <= 0.0) ? kfloor((x - val(Fo)) / (2.0 * val(K))) : kfloor((x + val(Fo)) / (2.0 * val(K))) - 1.0;
+      e.N = (val(beta) <= 0.0) ? kfloor((x - Fo) / (2.0 * K)) : kfloor((x + Fo) / (2.0 * K)) - 1.0;
     } else {
-      N = (val(beta) >= 0.0) ? kfloor((x + val(Fo)) / (2.0 * val(K))) : kfloor((x - val(Fo)) / (2.0 * val(K))) - 1.0;
+      e.N = (val(beta) >= 0.0) ? kfloor((x + Fo) / (2.0 * K)) : kfloor((x - Fo) / (2.0 * K)) - 1.0;
     }
   }
-  o.Imax = Im;
-  o.Nmax = static_cast<int>(N);
-  RadialConsts<T> rc;
-  radial_consts(c, px, false, rc);
-  JacobiLadder<T> lad;
-  jacobi_ladder(rc.v[R_k], lad);
-  const T rh = c.rplus;
-  // is_outside_crit (kerr_raytracing_utils.py:821-823)
-  const bool outside = (val(px.ri[3]) == 0.0) && (val(px.rr[3]) > val(rh)) && !(val(eta) < 0.0);
-  for (int m = 0; m <= mbar_max; ++m) {
-    T Ir;
-    {
+  radial_consts(c, px, false, e.rc);
+  jacobi_ladder(e.rc.v[R_k], e.lad);
+  e.outside = (val(px.ri[3]) == 0.0) && (val(px.rr[3]) > val(c.rplus)) && !(val(eta) < 0.0);
+}
+
+// I_r and r_s of subring m (equatorial_lensing.py:158-188): r_s = 0 where
+// N_max < m.  Needs a non-vortical ray.
+template <class T>
+KG_INLINE void eq_subring(const RayScalars<T>& c, const T& beta, const EqRay<T>& e, int m, T& Ir, T& rs) {
+  {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
-      double msub;
-      if (c.h > 0.0) {
-        msub = m + ((val(beta) > 0.0) ? 1.0 : 0.0);
-        Ir = (val(beta) <= 0.0) ? (2.0 * msub * K + Fo) / w : (2.0 * msub * K - Fo) / w;
-      } else {
-        msub = m + ((val(beta) < 0.0) ? 1.0 : 0.0);
-        Ir = (val(beta) >= 0.0) ? (2.0 * msub * K - Fo) / w : (2.0 * msub * K + Fo) / w;
-      }
+    double msub;
+    if (c.h > 0.0) {
+      msub = m + ((val(beta) > 0.0) ? 1.0 : 0.0);
+      Ir = (val(beta) <= 0.0) ? (2.0 * msub * e.K + e.Fo) / e.w : (2.0 * msub * e.K - e.Fo) / e.w;
+    } else {
+      msub = m + ((val(beta) < 0.0) ? 1.0 : 0.0);
+      Ir = (val(beta) >= 0.0) ? (2.0 * msub * e.K - e.Fo) / e.w : (2.0 * msub * e.K + e.Fo) / e.w;
     }
-    T rs = radial_eval(rc, lad, Ir, false).r;
-    if (N < m) rs = T(0.0);
+  }
+  rs = radial_eval(e.rc, e.lad, Ir, false).r;
+  if (e.N < m) rs = T(0.0);
+}
+
+// One pixel of the equatorial image for subrings 0..mbar_max.
+template <class T>
+KG_INLINE void equatorial_pixel(const RayScalars<T>& c, const T& alpha, const T& beta, const T& p_emis,
+                                const T& ri, int mbar_max, EqPixel<T>& o) {
+  o.image = T(0.0);
+  o.zero_mask = 0;
+  for (int m = 0; m < kMaxSubrings; ++m) {
+    o.I[m] = T(0.0);
+    o.g[m] = T(0.0);
+    o.r_s[m] = T(0.0);
+    o.Ir[m] = T(0.0);
+  }
+  EqRay<T> e;
+  eq_ray(c, alpha, beta, e);
+  if (e.vort) {  // vortical sentinels (equatorial_lensing.py:119-123)
+    for (int m = 0; m <= mbar_max; ++m) {
+      o.r_s[m] = T(-1.0);
+      o.Ir[m] = T(-1.0);
+      o.zero_mask |= 1 << m;
+    }
+    o.Imax = T(-1.0);
+    o.Nmax = -2;
+    return;
+  }
+  o.Imax = e.Im;
+  o.Nmax = static_cast<int>(e.N);
+  const T rh = c.rplus;
+  for (int m = 0; m <= mbar_max; ++m) {
+    T Ir, rs;
+    eq_subring(c, beta, e, m, Ir, rs);
     o.Ir[m] = Ir;
     o.r_s[m] = rs;
     // equatorial_images.py:175 (a NaN r_s is not masked there)
-    const bool zero = (N < m) || (N == -1.0) || (val(rs) <= val(rh));
+    const bool zero = (e.N < m) || (e.N == -1.0) || (val(rs) <= val(rh));
     if (zero) {
       o.zero_mask |= 1 << m;
       continue;
     }
-    const double kr = (!outside || val(Ir) <= 0.5 * val(Im)) ? 1.0 : -1.0;
+    const double kr = (!e.outside || val(Ir) <= 0.5 * val(e.Im)) ? 1.0 : -1.0;
     T u0, u1, u3;
     u_kep(c.a, ri, rs, u0, u1, u3);
-    T g = redshift(c.a, rs, lam, eta, kr, u0, u1, u3);
+    T g = redshift(c.a, rs, e.lam, e.eta, kr, u0, u1, u3);
     T I = (g * g * g) * kpow(rs, -p_emis);
     o.g[m] = g;
     o.I[m] = I;

@@ -18,6 +18,7 @@
 #include "kg_equatorial.hpp"
 #include "kg_geodesic.hpp"
 #include "kg_host_passes.hpp"
+#include "kg_lensing.hpp"
 #include "kg_sample.hpp"
 
 using namespace kg;
@@ -207,6 +208,20 @@ void kgh_counts(const double* sc, const double* alpha, const double* beta, int64
       if (turns_r) turns_r[o] = static_cast<int8_t>(radial_turns_at(rt, tj));
     }
   }
+}
+
+// rho_of_req (kg_lensing.hpp) on the CPU for one curve: rho[i], status[i]
+// at varphi[i]; sc as for the other entries (r_o = inf for the reference's).
+void kgh_rho_of_req(const double* sc, double req, int mbar, const double* varphi, int64_t n, double* rho,
+                    int32_t* status) {
+  RayScalars<double> c = scalars_from(sc);
+  for (int64_t i = 0; i < n; ++i) status[i] = rho_of_req_at(c, req, mbar, varphi[i], &rho[i]);
+}
+
+// r_equatorial2 (kg_lensing.hpp) on the CPU at screen points (rho[i], varphi[i]).
+void kgh_r_equatorial2(const double* sc, int mbar, const double* rho, const double* varphi, int64_t n, double* r) {
+  RayScalars<double> c = scalars_from(sc);
+  for (int64_t i = 0; i < n; ++i) r[i] = r_equatorial2(c, rho[i], std::cos(varphi[i]), std::sin(varphi[i]), mbar);
 }
 
 // Equatorial image (kg_equatorial.hpp) on the CPU: per-subring arrays are

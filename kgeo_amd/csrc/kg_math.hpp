@@ -87,6 +87,7 @@ KG_INLINE double frcp(double x) {
 #endif
 }
 KG_INLINE double kmax(double a, double b) { return a > b ? a : b; }
+KG_INLINE double kmin(double a, double b) { return a < b ? a : b; }
 KG_INLINE double parity_sign(double j) { return (::fmod(j, 2.0) == 0.0) ? 1.0 : -1.0; }
 
 template <class T> KG_INLINE T sq(const T& x) { return x * x; }

@@ -13,12 +13,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <mutex>
 
 #include "../../include/kgeo.h"
 #include "kg_dual.hpp"
 #include "kg_equatorial.hpp"
+#include "kg_lensing.hpp"
 
 using namespace kg;
 using Dual3 = kgd::Dual<3>;
@@ -148,6 +150,39 @@ __global__ __launch_bounds__(kBlock) void kg_eq_reduce(const double* __restrict_
   }
 }
 
+// One lane per (curve k, angle i): rho_of_req_single (equatorial_lensing.py:249-262)
+// for curve k's spin, inclination, target radius and subring at varphi[i].
+__global__ __launch_bounds__(kBlock) void kg_rho_of_req(const kgeo_ray_params* __restrict__ curves,
+                                                        const double* __restrict__ req,
+                                                        const int32_t* __restrict__ mbar, int64_t ncurves,
+                                                        const double* __restrict__ varphi, int64_t nphi,
+                                                        double* __restrict__ rho, int32_t* __restrict__ status) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (g >= ncurves * nphi) return;
+  const int64_t k = g / nphi, i = g - (g / nphi) * nphi;
+  const kgeo_ray_params& p = curves[k];
+  RayScalars<double> c = make_scalars(p.a, p.th_o, p.r_o, p.maxtaufrac);
+  c.a2 = p.a2;
+  c.sin_th = p.sin_th;
+  c.cos_th = p.cos_th;
+  c.cos2_th = p.cos2_th;
+  c.rplus = p.rplus;
+  c.rminus = p.rminus;
+  double r;
+  const int st = rho_of_req_at(c, req[k], mbar[k], varphi[i], &r);
+  rho[g] = r;
+  if (status) status[g] = st;
+}
+
+// One lane per photon-orbit radius: critical_curve (equatorial_lensing.py:264-293).
+__global__ __launch_bounds__(kBlock) void kg_critical_curve(double a, double sin_th, double tan_th, double cos_th,
+                                                            const double* __restrict__ r, int64_t n,
+                                                            double* __restrict__ alpha, double* __restrict__ beta) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  critical_point(a, sin_th, tan_th, cos_th, r[i], &alpha[i], &beta[i]);
+}
+
 int eq_fail(int code, const char* msg) {
   std::snprintf(g_err_eq, sizeof(g_err_eq), "%s", msg);
   return code;
@@ -260,6 +295,31 @@ int kgeo_equatorial_image(const kgeo_ray_params* p, const double* alpha, const d
   const unsigned nb = static_cast<unsigned>((p->npix + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(kg_eq_image, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), ep, alpha, beta, o);
   return eq_launch("kg_eq_image");
+}
+
+int kgeo_rho_of_req(const kgeo_ray_params* curves, const double* req, const int32_t* mbar, int64_t ncurves,
+                    const double* varphi, int64_t nphi, double* rho, int32_t* status, void* stream) {
+  if (ncurves < 0 || nphi < 0) return eq_fail(KGEO_E_ARG, "negative size");
+  if (ncurves == 0 || nphi == 0) return KGEO_OK;
+  if (!curves || !req || !mbar || !varphi || !rho) return eq_fail(KGEO_E_ARG, "NULL array");
+  if (ncurves > (int64_t(1) << 40) / nphi) return eq_fail(KGEO_E_ARG, "ncurves * nphi too large");
+  const int64_t n = ncurves * nphi;
+  const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(kg_rho_of_req, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), curves, req, mbar,
+                     ncurves, varphi, nphi, rho, status);
+  return eq_launch("kg_rho_of_req");
+}
+
+int kgeo_critical_curve(const kgeo_ray_params* p, const double* r, int64_t n, double* alpha, double* beta,
+                        void* stream) {
+  if (!p) return eq_fail(KGEO_E_ARG, "params is NULL");
+  if (n < 0) return eq_fail(KGEO_E_ARG, "n < 0");
+  if (n == 0) return KGEO_OK;
+  if (!r || !alpha || !beta) return eq_fail(KGEO_E_ARG, "NULL array");
+  const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(kg_critical_curve, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), p->a,
+                     p->sin_th, std::tan(p->th_o), p->cos_th, r, n, alpha, beta);
+  return eq_launch("kg_critical_curve");
 }
 
 int kgeo_equatorial_image_vjp(const kgeo_ray_params* p, const double* alpha, const double* beta, int mbar_max,

@@ -899,6 +899,72 @@ def r_equatorial(a, r_o, th_o, mbar, alpha, beta):
     return r_s, Ir, Imax, Nmax
 
 
+# ---------------------------------------------------------------------------
+# lensed curves of constant equatorial radius  (equatorial_lensing.py:198-293)
+# ---------------------------------------------------------------------------
+
+LENS_INF = 1.e50      # equatorial_lensing.py:18
+LENS_R0 = np.inf      # equatorial_lensing.py:19
+
+
+def r_equatorial2(rho, varphi, a, th0, mbar=0):
+    """r_s of subring mbar at the screen point rho (cos varphi, sin varphi),
+    1e50 for rays outside the critical curve with no such crossing, 1 for
+    rays inside it (equatorial_lensing.py:198-216)."""
+    alpha = np.cos(varphi) * rho
+    beta = np.sin(varphi) * rho
+    with np.errstate(all="ignore"):
+        r, Ir, Imax, Nmax = r_equatorial(a, LENS_R0, th0, mbar, np.array([alpha]), np.array([beta]))
+    r = r[0]
+    outside = is_outside_crit(a, th0, np.array([alpha]), np.array([beta]))[0]
+    if r <= 0 and outside == 1:
+        r = LENS_INF
+    elif r <= 0:
+        r = 1
+    return r
+
+
+def rho_bracket(req, mbar):
+    """The bracket rho_of_req searches (equatorial_lensing.py:229-238)."""
+    if mbar == 0:
+        return 0, req + 5
+    if mbar == 1:
+        return 0, 100
+    return 1, 10
+
+
+def rho_of_req_single(a, th0, req, varphi, mbar=0):
+    """brentq of r_equatorial2 - req over the bracket (equatorial_lensing.py:249-262)."""
+    from scipy.optimize import brentq
+    lo, hi = rho_bracket(req, mbar)
+    return brentq(lambda rho: r_equatorial2(rho, varphi, a, th0, mbar) - req, lo, hi)
+
+
+def rho_of_req(a, th0, req, mbar=0, varphis=None):
+    """(varphis, rhos, alphas, betas) over 360 angles by default
+    (equatorial_lensing.py:226-247)."""
+    if varphis is None:
+        varphis = np.linspace(-180, 179, 360) * np.pi / 180.
+    rhos = np.array([rho_of_req_single(a, th0, req, vp, mbar) for vp in varphis])
+    return varphis, rhos, np.cos(varphis) * rhos, np.sin(varphis) * rhos
+
+
+def critical_curve(a, th0, n=100000):
+    """Screen points of the bound photon orbits r in [r_-, r_+], NaNs
+    dropped, mirrored in beta (equatorial_lensing.py:264-293)."""
+    rminus = 2 * (1 + np.cos(2. * np.arccos(-a) / 3.))
+    rplus = 2 * (1 + np.cos(2. * np.arccos(a) / 3.))
+    r = np.linspace(rminus, rplus, n)
+    with np.errstate(invalid="ignore"):
+        lam = -(r ** 3 + a ** 2 + r * (a ** 2) - 3 * r ** 2) / (a * r - a)
+        eta = (r ** 2) * (3 * r ** 2 + a ** 2 - lam ** 2) / (r ** 2 - a ** 2)
+        alpha = -lam / np.sin(th0)
+        beta = np.sqrt(eta + (a * np.cos(th0)) ** 2 - (lam / np.tan(th0)) ** 2)
+    mask = np.isnan(alpha) + np.isnan(beta)
+    alpha, beta = alpha[~mask], beta[~mask]
+    return np.hstack((alpha, np.flip(alpha))), np.hstack((beta, -np.flip(beta)))
+
+
 def r_isco(a, retrograde=False):
     """ISCO radius (velocities.py:242-244)."""
     s = -1 if retrograde else 1
