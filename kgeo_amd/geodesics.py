@@ -129,39 +129,109 @@ class Geodesics(object):
     def sig_s(self):
         return self.affine_times
 
-    def plotgeos(self, xlim=12, nplot=None, ngeoplot=50):
-        """Minimal x-z projection of up to ngeoplot rays (the reference's
-        plotgeos, :95-199, draws the same projection with more decoration)."""
-        import matplotlib
-        matplotlib.use("Agg")
+    def plotgeos(self, xlim=12, rmax=15, nplot=None, ngeoplot=50, plot_disk=True, plot_inside_cc=True,
+                 plot_outside_cc=True, n_label="equatorial"):
+        """3D plot of the rays coloured by their number of equatorial crossings
+        (n_label='poloidal': by completed poloidal orbits), over the horizon
+        sphere and the equatorial disk, as kerr_raytracing_utils.py:95-199
+        draws it: rays with label j use colour j of (dimgrey, b, g, orange, r,
+        m, c, y); at most ngeoplot rays per label for labels below 4, each
+        drawn where r < rmax or before half its Mino time, and within 3 rmax.
+        Returns the figure."""
         import matplotlib.pyplot as plt
-        r, th, ph = self.r_s, self.th_s, self.ph_s
-        idx = np.linspace(0, self.npix - 1, min(ngeoplot, self.npix)).astype(int)
+        a, th_o = self.a, self.th_o
+        r_s, th_s, ph_s, tausteps = self.r_s, self.th_s, self.ph_s, self.tausteps
+        if n_label == "poloidal":
+            nmax_eq = np.floor(self.n_poloidal[-1])
+        else:
+            nmax_eq = np.array(self.nmax_eq, dtype=float)
+        nmax_eq = nmax_eq + 1
+        rplus = 1 + np.sqrt(1 - a ** 2)
+        x_s = r_s * np.cos(ph_s) * np.sin(th_s)
+        y_s = r_s * np.sin(ph_s) * np.sin(th_s)
+        z_s = r_s * np.cos(th_s)
+
         fig = plt.figure()
-        ax = fig.add_subplot(111)
-        for i in idx:
-            x = r[:, i] * np.sin(th[:, i]) * np.cos(ph[:, i])
-            z = r[:, i] * np.cos(th[:, i])
-            ax.plot(x, z, lw=0.5)
+        ax = fig.add_subplot(projection="3d")
+        u, v = np.mgrid[0:2 * np.pi:30j, 0:np.pi:20j]
+        ax.plot_surface(rplus * np.cos(u) * np.sin(v), rplus * np.sin(u) * np.sin(v), rplus * np.cos(v), color="black")
+        if plot_disk:
+            rr, pp = np.mgrid[0:xlim, 0:2 * np.pi:20j]
+            ax.plot_surface(rr * np.cos(pp), rr * np.sin(pp), np.zeros(rr.shape), alpha=0.5)
         ax.set_xlim(-xlim, xlim)
         ax.set_ylim(-xlim, xlim)
-        ax.set_aspect("equal")
+        ax.set_zlim(-xlim, xlim)
+        ax.auto_scale_xyz([-xlim, xlim], [-xlim, xlim], [-xlim, xlim])
+        ax.set_axis_off()
+        rmax2 = 3 * rmax
+        ax.plot3D([0, rmax2 * np.sin(th_o)], [0, 0], [0, rmax2 * np.cos(th_o)], "black", ls="dashed")
+
+        colors = ["dimgrey", "b", "g", "orange", "r", "m", "c", "y"]
+        maxwraps = int(np.nanmax(nmax_eq))
+        print("maxwraps ", maxwraps)
+        if nplot is None:
+            nplot = range(min(maxwraps + 1, len(colors)))
+        else:
+            nplot = np.array([nplot]).flatten()
+        for j in nplot:
+            mask = ((nmax_eq == 0) | (nmax_eq == -1)) if j == 0 else (nmax_eq == j)
+            if not plot_inside_cc:
+                mask = mask & (r_s[-1] > 10)
+            if not plot_outside_cc:
+                mask = mask & (r_s[-1] < 10)
+            if not np.any(mask):
+                continue
+            xs, ys, zs = x_s[:, mask], y_s[:, mask], z_s[:, mask]
+            rs, tau = r_s[:, mask], tausteps[:, mask]
+            trim = max(int(xs.shape[-1] / ngeoplot), 1) if ngeoplot is not None else 0
+            if xs.shape[-1] < 5 or j >= 4 or trim == 0:
+                geos = range(xs.shape[-1])
+            else:
+                geos = range(0, xs.shape[-1], trim)
+            for i in geos:
+                keep = ((rs[:, i] < rmax) | (tau[:, i] < .5 * tau[-1, i])) & (rs[:, i] < rmax2)
+                ax.plot3D(xs[keep, i], ys[keep, i], zs[keep, i], colors[j])
         return fig
 
-    def savegeos(self, path="./"):
-        """HDF5 output with the reference's keys (:201-222); needs h5py."""
-        import h5py  # noqa: F401  (raises ImportError where h5py is absent)
-        fname = path + "a%0.2f_th%0.2f_geo.h5" % (self.a, self.th_o * 180 / np.pi)
-        with h5py.File(fname, "w") as f:
-            f.create_dataset("spin", data=self.a)
-            f.create_dataset("inc", data=self.th_o)
-            f.create_dataset("r_o", data=self.r_o)
-            f.create_dataset("alpha", data=self.alpha)
-            f.create_dataset("beta", data=self.beta)
-            f.create_dataset("t", data=self.t_s)
-            f.create_dataset("r", data=self.r_s)
-            f.create_dataset("theta", data=self.th_s)
-            f.create_dataset("phi", data=self.ph_s)
-            f.create_dataset("affine", data=self.sig_s)
-            f.create_dataset("mino", data=self.mino_times)
+    def savegeos(self, outfile=None, path="./"):
+        """HDF5 file with the reference's datasets (kerr_raytracing_utils.py:
+        201-222): scalars spin, inc, r_o; alpha, beta (npix); t, r, theta,
+        phi, affine, mino (ngeo, npix).  Default name
+        path + 'a%0.2f_th%0.2f_geo.h5' % (a, th_o in degrees).  Written with
+        h5py when it is installed, else by h5lite (the same datasets in the
+        HDF5 format's original layout).  Returns the file name."""
+        fname = outfile if outfile is not None else path + "a%0.2f_th%0.2f_geo.h5" % (self.a, self.th_o * 180 / np.pi)
+        data = {"spin": self.a, "inc": self.th_o, "r_o": self.r_o, "alpha": self.alpha, "beta": self.beta,
+                "t": self.t_s, "r": self.r_s, "theta": self.th_s, "phi": self.ph_s, "affine": self.sig_s,
+                "mino": self.tausteps}
+        try:
+            import h5py
+        except ImportError:
+            h5py = None
+        if h5py is None:
+            from . import h5lite
+            h5lite.write(fname, data)
+        else:
+            with h5py.File(fname, "w") as f:
+                for k, v in data.items():
+                    f.create_dataset(k, data=v)
         return fname
+
+
+def loadgeos(infile):
+    """Geodesics from a savegeos file (kerr_raytracing_utils.py:224-241),
+    read with h5py when it is installed, else by h5lite."""
+    keys = ("spin", "r_o", "inc", "alpha", "beta", "mino", "affine", "t", "r", "theta", "phi")
+    try:
+        import h5py
+    except ImportError:
+        h5py = None
+    if h5py is None:
+        from . import h5lite
+        hf = h5lite.read(infile)
+        d = {k: hf[k] for k in keys}
+    else:
+        with h5py.File(infile, "r") as hf:
+            d = {k: hf[k][()] for k in keys}
+    return Geodesics(d["spin"], [0, d["r_o"], d["inc"], 0], [d["alpha"], d["beta"]], d["mino"], d["affine"],
+                     [d["t"], d["r"], d["theta"], d["phi"]])
