@@ -124,6 +124,36 @@ def test_batch_refuses_surveys_without_phi_t():
         BatchSurvey(images, r_o=R_O)
 
 
+@pytest.mark.skip(reason="surveys without phi and t are refused (KgeoError, see "
+                         "test_batch_refuses_surveys_without_phi_t): a batch kernel variant with per-survey "
+                         "phi/t instantiations ended a GPU run in a fault whose cause was not shown, so the "
+                         "no-phi/t kernels were taken out instead of being run")
+def test_batch_without_phi_t_and_mixed_surveys():
+    """Surveys where no image, or only some images, want phi and t: r and
+    theta of every image, and t and phi of the images that want them, equal
+    plain calls with the same per-image flag to the walk's rounding."""
+    for mixed in (False, True):
+        images = _images()
+        for k, im in enumerate(images):
+            im["do_phi_and_t"] = mixed and k % 2 == 0
+        sv = _batch(images)
+        arena_p, labels_p = _plain(images, sv.out_off)
+        for k in ("rclass", "thclass", "nmax_eq", "status"):
+            assert _same(_host(getattr(sv, k)), labels_p[k]), (mixed, k)
+        for key, a_b in sv.arena.items():
+            a_b = _host(a_b)
+            for k, im in enumerate(images):
+                if key in ("t", "ph") and not im["do_phi_and_t"]:
+                    continue
+                o0, o1 = int(sv.out_off[k]), int(sv.out_off[k + 1])
+                x, y = a_b[o0:o1], arena_p[key][o0:o1]
+                assert np.array_equal(np.isnan(x), np.isnan(y)), (mixed, key, k)
+                ok = ~np.isnan(y)
+                rel = np.abs(x[ok] - y[ok]) / np.maximum(np.abs(y[ok]), 1.0)
+                assert rel.max(initial=0.0) < 4e-11, (mixed, key, k, float(rel.max()))
+        sv.close()
+
+
 @pytest.mark.parametrize("k", [0, 1, 6])
 def test_batch_images_match_oracle(k):
     images = _images()
