@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define KGEO_ABI_VERSION 3
+#define KGEO_ABI_VERSION 4
 
 /* return codes */
 #define KGEO_OK 0
@@ -240,6 +240,129 @@ int kgeo_rho_of_req(const kgeo_ray_params* curves, const double* req, const int3
  */
 int kgeo_critical_curve(const kgeo_ray_params* p, const double* r, int64_t n, double* alpha, double* beta,
                         void* stream);
+
+/*
+ * Emission models of the equatorial disk: the reference's Velocity
+ * (velocities.py:23-176), Bfield (bfields.py:33-427) and Emissivity
+ * (emissivities.py:37-118) objects as plain descriptors.  Each field is the
+ * reference's constructor argument of the same name, with its default
+ * filled in by the caller (kgeo_amd.models does so).  Tables are device
+ * arrays of 4 x n doubles, column blocks [r | c1 | c2 | c3] with r
+ * increasing, interpolated as np.interp does.
+ */
+#define KGEO_VEL_ZAMO 0
+#define KGEO_VEL_INFALL 1
+#define KGEO_VEL_KEP 2        /* 'kep', 'cunningham' */
+#define KGEO_VEL_SUBKEP 3     /* 'subkep', 'cunningham_subkep' */
+#define KGEO_VEL_GENERAL 4
+#define KGEO_VEL_GELLES 5
+#define KGEO_VEL_SIMFIT 6
+#define KGEO_VEL_FROMFILE 7   /* table: r, U1, U2, U3 */
+#define KGEO_VEL_DRIFTFRAME 8 /* numeric nu_parallel, no gammamax */
+
+#define KGEO_BF_SIMPLE 0      /* 'rad', 'vert', 'tor', 'simple' */
+#define KGEO_BF_SIMPLE_RM1 1
+#define KGEO_BF_MONOPOLE_A 2
+#define KGEO_BF_BZ_MONOPOLE 3
+#define KGEO_BF_BZ_GUESS 4
+#define KGEO_BF_BZ_PARA 5
+#define KGEO_BF_POWER 6
+#define KGEO_BF_GEN_POWER 7
+#define KGEO_BF_FROMFILE 8    /* table: r, Br, Btheta, Bphi */
+
+#define KGEO_EMIS_CONSTANT 0
+#define KGEO_EMIS_BPL 1
+#define KGEO_EMIS_GLM 2       /* 'glm' and 'ring' (mu_ring > 0) */
+#define KGEO_EMIS_THERMAL 3
+
+typedef struct kgeo_bfield_model {
+  int32_t kind;             /* KGEO_BF_* */
+  int32_t secondorder_only; /* bz_monopole */
+  int32_t usemono;          /* power */
+  int32_t is_above;         /* gen_power isAbove */
+  double C;                 /* bz_monopole, bz_guess, bz_para, monopoleA, power */
+  double Cr, Cvert, Cph;    /* simple, simple_rm1 */
+  double omegafac;          /* monopoleA */
+  double p;                 /* power */
+  double n_I, p_val;        /* gen_power */
+  const double* table;      /* fromfile */
+  int64_t table_n;
+} kgeo_bfield_model;
+
+typedef struct kgeo_velocity_model {
+  int32_t kind;             /* KGEO_VEL_* */
+  int32_t retrograde;       /* kep, subkep, general */
+  double fac_subkep, beta_phi, beta_r;  /* subkep, general */
+  double gelles_beta, gelles_chi;       /* gelles */
+  double ell_isco, vr_isco, p1, p2, dd; /* simfit */
+  double nu_parallel;                   /* driftframe */
+  kgeo_bfield_model drift_field;        /* driftframe: the field it drifts in */
+  const double* table;                  /* fromfile */
+  int64_t table_n;
+} kgeo_velocity_model;
+
+typedef struct kgeo_emissivity_model {
+  int32_t kind;             /* KGEO_EMIS_* */
+  int32_t bfield_model;     /* thermal: take |B| from the field model */
+  double emiscut_in, emiscut_out;
+  double p1, p2;            /* bpl */
+  double mu_ring;           /* glm: 0 means 1 - sqrt(1 - a^2); ring: r_ring */
+  double gamma_off, sigma;  /* glm, ring */
+  double Rb, ne0, Te0, B0, alpha_n, alpha_T, alpha_B; /* thermal */
+} kgeo_emissivity_model;
+
+typedef struct kgeo_emission_model {
+  kgeo_velocity_model velocity;
+  kgeo_bfield_model bfield;
+  kgeo_emissivity_model emissivity;
+  int32_t polarization, pathlength;     /* Iobs flags */
+  double specind, diskangle, nu_obs;
+} kgeo_emission_model;
+
+/*
+ * Replaces make_image / Iobs (equatorial_images.py:23-264) with
+ * calc_redshift (:307-334), calc_polquantities (:336-487), calc_evpa
+ * (:524-544) and calc_pathlength_equatorial (:489-522), for subrings
+ * mbar = 0..mbar_max in one launch (one lane per pixel).  Emission is
+ * evaluated at theta = pi/2 as numpy rounds it.  Outputs are [npix][nsub]
+ * (nsub = mbar_max + 1, the reference's (npix, mbar_max+1) arrays):
+ *   I, Q, U      observed Stokes intensities (0 where masked; Q = U = 0
+ *                without polarization)
+ *   g, sin_thb   redshift and sin(theta_B) (0 where masked; sin_thb is pi/4
+ *                without polarization, as the reference assumes)
+ *   r_s, Ir      emission radius and Mino time as r_equatorial returns them
+ *   lp, Ie       with pathlength: path length and rest-frame emission
+ * and per pixel: nmax_eq (N_max of r_equatorial, -2 for eta <= 0) and
+ * n_poloidal (nmax_poloidal: n_poloidal_orbits at the total Mino time).
+ * Every output may be NULL.  r_o must be the reference's (>= 100 or inf).
+ * Models the reference cannot image return KGEO_E_ARG: a comoving field,
+ * pathlength with a velocity that has u^theta != 0 (driftframe).
+ */
+int kgeo_equatorial_stokes(const kgeo_ray_params* p, const double* alpha, const double* beta, int32_t mbar_max,
+                           const kgeo_emission_model* model, double* I, double* Q, double* U, double* g,
+                           double* r_s, double* sin_thb, double* Ir, double* nmax_eq, double* n_poloidal,
+                           double* lp, double* Ie, void* stream);
+
+/*
+ * The models at given points, replacing Velocity.u_lab (velocities.py:70-96),
+ * Velocity.tetrades (:124-176), Bfield.bfield_lab / omega_field / bsq
+ * (bfields.py:93-131, :175-196, :365-395) and Emissivity.jrest
+ * (emissivities.py:83-118).  For point i at radius r[i] and polar angle
+ * th[i] (th NULL: pi/2 as numpy rounds it):
+ *   u [4][n]      contravariant four-velocity (model->velocity)
+ *   tetrad [16][n] e_(a)^mu, rows (t, x, y, z) x columns mu (velocity)
+ *   B [3][n]      lab-frame B^r, B^theta, B^phi (model->bfield)
+ *   omega [n]     field-line angular velocity (model->bfield)
+ *   bsq [n]       fluid-frame b^2 of model->bfield in model->velocity
+ *   j [n]         jrest(a, r, g, sinthetab, Bmag, nu_obs, specind) of
+ *                 model->emissivity; g, sinthb, bmag [n] may be NULL (None)
+ * Every output may be NULL.  Models whose quantity the reference does not
+ * define (omega of a simple field) return KGEO_E_ARG.
+ */
+int kgeo_model_eval(const kgeo_emission_model* model, double a, const double* r, const double* th, int64_t n,
+                    const double* g, const double* sinthb, const double* bmag, double* u, double* tetrad, double* B,
+                    double* omega, double* bsq, double* j, void* stream);
+const char* kgeo_emission_last_error(void);
 
 /*
  * No counterpart in the reference.  kgeo_equatorial_image_vjp without a

@@ -12,7 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # must equal KGEO_ABI_VERSION of include/kgeo.h
-ABI_VERSION = 3
+ABI_VERSION = 4
 LIB_PATH = os.path.join(_HERE, "libkgeo_hip.so")
 
 KGEO_DO_PHI_T = 1
@@ -46,6 +46,41 @@ class BatchTuning(ctypes.Structure):
 
     _fields_ = [("samples_per_item", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
                 ("reserved", ctypes.c_int32 * 2)]
+
+
+class BfieldModel(ctypes.Structure):
+    """kgeo_bfield_model (include/kgeo.h)."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("secondorder_only", ctypes.c_int32), ("usemono", ctypes.c_int32),
+                ("is_above", ctypes.c_int32), ("C", ctypes.c_double), ("Cr", ctypes.c_double),
+                ("Cvert", ctypes.c_double), ("Cph", ctypes.c_double), ("omegafac", ctypes.c_double),
+                ("p", ctypes.c_double), ("n_I", ctypes.c_double), ("p_val", ctypes.c_double),
+                ("table", ctypes.c_void_p), ("table_n", ctypes.c_int64)]
+
+
+class VelocityModel(ctypes.Structure):
+    """kgeo_velocity_model (include/kgeo.h)."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("retrograde", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("fac_subkep", "beta_phi", "beta_r", "gelles_beta", "gelles_chi", "ell_isco",
+                                       "vr_isco", "p1", "p2", "dd", "nu_parallel")] + [
+        ("drift_field", BfieldModel), ("table", ctypes.c_void_p), ("table_n", ctypes.c_int64)]
+
+
+class EmissivityModel(ctypes.Structure):
+    """kgeo_emissivity_model (include/kgeo.h)."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("bfield_model", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in ("emiscut_in", "emiscut_out", "p1", "p2", "mu_ring", "gamma_off", "sigma", "Rb",
+                                       "ne0", "Te0", "B0", "alpha_n", "alpha_T", "alpha_B")]
+
+
+class EmissionModel(ctypes.Structure):
+    """kgeo_emission_model (include/kgeo.h)."""
+
+    _fields_ = [("velocity", VelocityModel), ("bfield", BfieldModel), ("emissivity", EmissivityModel),
+                ("polarization", ctypes.c_int32), ("pathlength", ctypes.c_int32), ("specind", ctypes.c_double),
+                ("diskangle", ctypes.c_double), ("nu_obs", ctypes.c_double)]
 
 
 class KgeoError(RuntimeError):
@@ -85,6 +120,12 @@ _PROTOS = {
     "kgeo_equatorial_image_vjp_noalloc": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p,
                                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
                                           + [ctypes.c_void_p] * 4),
+    "kgeo_equatorial_stokes": (ctypes.c_int, [ctypes.POINTER(RayParams), ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_int32, ctypes.POINTER(EmissionModel)]
+                               + [ctypes.c_void_p] * 12),
+    "kgeo_model_eval": (ctypes.c_int, [ctypes.POINTER(EmissionModel), ctypes.c_double, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 10),
+    "kgeo_emission_last_error": (ctypes.c_char_p, []),
     "kgeo_crossings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
                        + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "kgeo_turnings": (ctypes.c_int, [ctypes.POINTER(RayParams)] + [ctypes.c_void_p] * 3
@@ -130,7 +171,9 @@ def load():
 def check(rc, what):
     if rc != 0:
         lib = load()
-        if what.startswith("kgeo_equatorial"):
+        if what in ("kgeo_equatorial_stokes", "kgeo_model_eval"):
+            err = lib.kgeo_emission_last_error
+        elif what.startswith("kgeo_equatorial") or what in ("kgeo_rho_of_req", "kgeo_critical_curve"):
             err = lib.kgeo_equatorial_last_error
         elif what.startswith("kgeo_batch"):
             err = lib.kgeo_batch_last_error

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["kgeo_raytrace.hip", "kgeo_equatorial.hip", "kgeo_batch.hip"]
+HIP_SOURCES = ["kgeo_raytrace.hip", "kgeo_equatorial.hip", "kgeo_batch.hip", "kgeo_emission.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of every kernel

@@ -15,6 +15,7 @@
 
 #include "kg_counts.hpp"
 #include "kg_dual.hpp"
+#include "kg_emission.hpp"
 #include "kg_equatorial.hpp"
 #include "kg_geodesic.hpp"
 #include "kg_host_passes.hpp"
@@ -259,6 +260,40 @@ void kgh_equatorial_jac(double a, double th_o, double r_o, double p, const doubl
     equatorial_pixel(c, D(alpha[i]), D(beta[i]), pd, ri, mbar_max, e);
     image[i] = e.image.v;
     for (int k = 0; k < 3; ++k) jac[k * n + i] = e.image.d[k];
+  }
+}
+
+// Polarized image (kg_emission.hpp) on the CPU, outputs [n][nsub] as
+// kgeo_equatorial_stokes writes them.
+void kgh_stokes(const double* sc, const kgeo_emission_model* M, const double* alpha, const double* beta, int64_t n,
+                int nsub, double* I, double* Q, double* U, double* g, double* rs, double* sinthb, double* Ir,
+                double* nmax, double* npol, double* lp, double* Ie) {
+  RayScalars<double> c = scalars_from(sc);
+  const bool north = sc[2] < kHalfPiD;
+  const double bsq0 =
+      (M->emissivity.kind == KGEO_EMIS_THERMAL) ? field_bsq(M->bfield, M->velocity, c.a, M->emissivity.Rb, equator()) : 0.0;
+  StokesArrays o{I, Q, U, g, rs, sinthb, Ir, nmax, npol, lp, Ie};
+  for (int64_t i = 0; i < n; ++i) stokes_pixel(c, north, *M, alpha[i], beta[i], nsub, bsq0, i, o);
+}
+
+// The emission models at points (kgeo_model_eval on the CPU); th NULL is the
+// equator.
+void kgh_model_eval(const kgeo_emission_model* M, double a, const double* r, const double* th, int64_t n,
+                    const double* g, const double* sinthb, const double* bmag, double* u, double* tet, double* B,
+                    double* omega, double* bsq, double* j) {
+  for (int64_t i = 0; i < n; ++i) {
+    const Angle an = th ? angle_of(th[i]) : equator();
+    model_point(*M, a, r[i], an, i, n, g, sinthb, bmag, u, tet, B, omega, bsq, j);
+  }
+}
+
+// Special functions of the field models: which 0 = Li2(x) (x <= 1),
+// 1 = Lambert W0(x), 2 = BZ f(r) with f'(r) in out2.
+void kgh_special(int which, const double* x, int64_t n, double* out, double* out2) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (which == 0) out[i] = li2(x[i]);
+    else if (which == 1) out[i] = lambert_w0(x[i]);
+    else bz_f(x[i], out[i], out2[i]);
   }
 }
 

@@ -140,6 +140,38 @@ class Host:
     def legendre(self, n, phi, m):
         return self._call("kgh_legendre", [n, phi, m], 3)
 
+    def stokes(self, a, r_o, th_o, model, alpha, beta, nsub):
+        """kg_emission.hpp's stokes_pixel on the CPU: dict of (n, nsub) arrays
+        I, Q, U, g, r_s, sin_thb, Ir, lp, Ie and (n,) nmax_eq, n_poloidal."""
+        alpha = np.ascontiguousarray(alpha, float)
+        beta = np.ascontiguousarray(beta, float)
+        n = len(alpha)
+        names = ("I", "Q", "U", "g", "r_s", "sin_thb", "Ir", "nmax_eq", "n_poloidal", "lp", "Ie")
+        out = {k: np.full((n,) if k in ("nmax_eq", "n_poloidal") else (n, nsub), np.nan) for k in names}
+        self.lib.kgh_stokes(self._p(self.scalars(a, r_o, th_o)), ctypes.byref(model), self._p(alpha), self._p(beta),
+                            ctypes.c_int64(n), ctypes.c_int(nsub), *[self._p(out[k]) for k in names])
+        return out
+
+    def model_eval(self, model, a, r, th=None, g=None, sinthb=None, bmag=None):
+        """kg_emission.hpp's model_point on the CPU: dict u (4, n), tetrad
+        (16, n), B (3, n), omega, bsq (n,) and, when the emissivity is
+        evaluable, j (n,)."""
+        r = np.ascontiguousarray(r, float)
+        n = len(r)
+        out = {"u": np.empty((4, n)), "tetrad": np.empty((16, n)), "B": np.empty((3, n)), "omega": np.empty(n),
+               "bsq": np.empty(n), "j": np.empty(n)}
+        P = lambda x: None if x is None else self._p(np.ascontiguousarray(x, float))
+        self.lib.kgh_model_eval(ctypes.byref(model), ctypes.c_double(a), self._p(r), P(th), ctypes.c_int64(n), P(g),
+                                P(sinthb), P(bmag), *[self._p(out[k]) for k in ("u", "tetrad", "B", "omega", "bsq",
+                                                                                "j")])
+        return out
+
+    def special(self, which, x):
+        x = np.ascontiguousarray(x, float)
+        out, out2 = np.empty(len(x)), np.empty(len(x))
+        self.lib.kgh_special(ctypes.c_int(which), self._p(x), ctypes.c_int64(len(x)), self._p(out), self._p(out2))
+        return out, out2
+
     @staticmethod
     def scalars(a, r_o, th_o, maxtaufrac=O.MAXTAUFRAC):
         return np.array([a, r_o, th_o, maxtaufrac, a ** 2, np.sin(th_o), np.cos(th_o), np.cos(th_o) ** 2,

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_load_and_version():
     lib = _lib.load()
-    assert lib.kgeo_abi_version() == 3
+    assert lib.kgeo_abi_version() == 4
     assert _lib.workspace_bytes(1000) == 256 + 1000 * _plan_slots() * 8
 
 
