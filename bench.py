@@ -161,13 +161,16 @@ def cpu_baseline(th):
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kg_sample_passes_512x512_ngeo500_pmc.txt")
 
 
-def read_profile_value(key):
-    """A figure from the committed rocprofv3 counter summary of the sample
-    passes at this workload (FETCH_SIZE + WRITE_SIZE scaled by a plain
-    copy's reading; hardware FP64 FLOP count), or None."""
-    if not os.path.exists(PMC_SUMMARY):
+IMAGE_SUMMARY = os.path.join(ROOT, "profiles", "kg_stokes_polarized_800x800_trace_pmc.md")
+
+
+def read_profile_value(key, path=PMC_SUMMARY):
+    """A figure from a committed rocprofv3 counter summary (by default the
+    sample passes at the raytrace workload: FETCH_SIZE + WRITE_SIZE scaled by
+    a plain copy's reading; hardware FP64 FLOP count), or None."""
+    if not os.path.exists(path):
         return None
-    for ln in open(PMC_SUMMARY):
+    for ln in open(path):
         if ln.startswith(key + ":"):
             return float(ln.split(":", 1)[1])
     return None
@@ -359,6 +362,88 @@ def grad_image_rate(steps=20, warmup=3):
                         "alpha,beta in [-12,12] (bench.py --mode equatorial for the 1024^2 image)"}
 
 
+IMAGE_WORKLOAD = ("polarized make_image 800x800, subrings 0..1 (a=0.7, th_o=45 deg, r_o=inf, alpha,beta in "
+                  "[-10,10]; general velocity fac_subkep=0.6 beta_r=0.7 beta_phi=0.8, bz_guess field, ring "
+                  "emissivity, specind 1): the reference's tests/test_images.py case")
+
+
+def polarized_image_figures(steps=5, warmup=2, cpu_pixels=2000):
+    """kgeo_equatorial_stokes on the reference's image test (IMAGE_WORKLOAD):
+    kernel time by HIP events on the launching stream, pixels x subrings per
+    second, and the oracle's rate on a seeded pixel sample (one core)."""
+    import torch
+    from kgeo_amd import images
+    from kgeo_amd import models as M
+
+    a, th, nx = 0.7, 45.0 * np.pi / 180.0, 800
+    A, B = images.pixel_grid(-10., 10., -10., 10., 20. / nx)
+    dev = torch.device("cuda", 0)
+    al, be = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    vel = M.Velocity("general", fac_subkep=0.6, beta_r=0.7, beta_phi=0.8)
+    bf, em = M.Bfield("bz_guess", C=1.), M.Emissivity("ring")
+
+    import ctypes
+    from kgeo_amd import _lib
+    from kgeo_amd.raytrace import _ptr, _stream_handle
+    lib = _lib.load()
+    model = M.emission_model(vel, bf, em, polarization=True, specind=1.)
+    prm = _lib.make_params(a, np.inf, th, len(A), 1, 1.0)
+    outs = [torch.empty((len(A), 2), device=dev, dtype=torch.float64) for _ in range(3)]
+    stream = _stream_handle()
+
+    def run():   # one kgeo_equatorial_stokes launch, arguments prepared once
+        _lib.check(lib.kgeo_equatorial_stokes(prm, _ptr(al), _ptr(be), 1, ctypes.byref(model),
+                                              *[_ptr(o) for o in outs], *([None] * 8), stream),
+                   "kgeo_equatorial_stokes")
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for e0, e1 in evs:
+        e0.record()
+        run()
+        e1.record()
+    torch.cuda.synchronize()
+    ref = images.stokes_device(a, np.inf, th, 1, al, be, emissivity=em, velocity=vel, bfield=bf, polarization=True,
+                               specind=1., want=("I",))
+    torch.cuda.synchronize()
+    assert torch.equal(ref["I"], outs[0])
+    ms = float(np.median([e0.elapsed_time(e1) for e0, e1 in evs]))
+    units = len(A) * 2
+    out = {"workload": IMAGE_WORKLOAD, "ms_per_image": ms, "pixel_subrings_per_s": units / (ms * 1e-3),
+           "steps": steps}
+    hw = read_profile_value("hardware_fp64_flops_per_launch", IMAGE_SUMMARY)
+    if hw:
+        tf = hw / (ms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu-fp64", "kernel": "kg_stokes", "achieved": tf, "peak": FP64_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS, "flops": "hardware FP64 count",
+                           "hardware_fp64_flops_per_launch": hw, "traffic": 64.0 * len(A)}
+    try:
+        from oracle import kgeo_oracle_emission as E
+        rng = np.random.default_rng(0)
+        idx = rng.choice(len(A), cpu_pixels, replace=False)
+        ev = E.Velocity("general", fac_subkep=0.6, beta_r=0.7, beta_phi=0.8)
+        t0 = time.perf_counter()
+        E.make_image_pixels(a, np.inf, th, 1, A[idx], B[idx], emissivity=E.Emissivity("ring"), velocity=ev,
+                            bfield=E.Bfield("bz_guess", C=1.), polarization=True, specind=1.)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle"] = {"pixel_subrings_per_s": cpu_pixels * 2 / dt, "cores": 1, "kind": "port",
+                             "sample": f"{cpu_pixels} seeded pixels of the same image (numpy, vectorised)"}
+    except ImportError:
+        out["cpu_oracle"] = None
+    log(f"polarized image: {ms:.3f} ms/image ({units / (ms * 1e-3):.3e} pixel-subrings/s)")
+    return out
+
+
+def bench_image(args):
+    f = polarized_image_figures(args.steps, args.warmup)
+    print(json.dumps({"metric": "polarized image pixel-subrings/s", "value": f["pixel_subrings_per_s"],
+                      "unit": "pixel-subrings/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": f["ms_per_image"], "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                      "config": {"workload": IMAGE_WORKLOAD}, "image": f}), flush=True)
+
+
 def bench_copy(args):
     """A plain device-to-device copy of known size (1 GiB read + 1 GiB
     written per launch): the reference point that scales the FETCH_SIZE and
@@ -388,7 +473,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="leave the batched-survey figures out of the line")
-    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial", "copy"), default="raytrace")
+    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial", "image", "copy"), default="raytrace")
     ap.add_argument("--retune", action="store_true", help="batch mode: re-run the autotuner")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -402,6 +487,8 @@ def main():
         return bench_equatorial(args)
     if args.mode == "copy":
         return bench_copy(args)
+    if args.mode == "image":
+        return bench_image(args)
 
     import torch
     from kgeo_amd import _lib
@@ -477,6 +564,7 @@ def main():
 
     grad = grad_image_rate()
     log(f"grad-images: {grad['ms_per_grad_image']:.3f} ms -> {grad['value']:.0f} grad-images/s")
+    image = polarized_image_figures()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -496,7 +584,8 @@ def main():
                                    "alpha,beta in [-8,8]); writes r, theta, phi, t",
                        "npix": npix, "ngeo": NGEO, "parallelism": "single GPU, pixels x sample chunks"},
             "roofline": roofline, "cpu_baseline": cpu,
-            "gpu_vs_cpu": (value / cpu["value"]) if cpu else None, "grad_images": grad, "batch": batch,
+            "gpu_vs_cpu": (value / cpu["value"]) if cpu else None, "grad_images": grad, "polarized_image": image,
+            "batch": batch,
             "nonfinite_pixels": nonfinite, "host": platform.node()}
     print(json.dumps(line), flush=True)
 

@@ -55,8 +55,8 @@ def rho_of_req_device(spins, incs, reqs, mbars, varphis):
     spins = np.atleast_1d(np.asarray(spins, dtype=np.float64))
     n = len(spins)
     incs = np.broadcast_to(np.asarray(incs, dtype=np.float64), (n,))
-    reqs = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(reqs, np.float64), (n,)))).to(dev)
-    mbars = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(mbars, np.int32), (n,)))).to(dev)
+    reqs = torch.from_numpy(np.array(np.broadcast_to(np.asarray(reqs, np.float64), (n,)))).to(dev)
+    mbars = torch.from_numpy(np.array(np.broadcast_to(np.asarray(mbars, np.int32), (n,)))).to(dev)
     vp = torch.from_numpy(np.ascontiguousarray(varphis, dtype=np.float64)).to(dev)
     curves = _curve_params(spins, incs)
     rho = torch.empty((n, vp.numel()), device=dev, dtype=torch.float64)
