@@ -150,12 +150,21 @@ def cpu_baseline(th):
                 break
     except Exception:
         pass
+    try:   # /proc/cpuinfo carries the full name where lscpu shortens it
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                full = ln.split(":", 1)[1].strip()
+                model = full if len(full) > len(model) or model == "unknown" else model
+                break
+    except OSError:
+        pass
     head = runs[f"c2_subset_{procs}p"]
     return {"value": head["samples_per_s"], "unit": "samples/s", "cores": procs, "kind": "port",
             "sample": "oracle on the seeded 4096-pixel subset of the 512x512 ngeo=500 image (2,048,000 "
                       f"samples), {procs} processes x 1 thread, median of 3 after a warm-up",
             "single_process_value": runs["c2_subset_1p"]["samples_per_s"], "runs": runs,
-            "cpu_model": model, "numpy": np.__version__, "scipy": scipy.__version__}
+            "cpu_model": model, "affinity_cpus": len(os.sched_getaffinity(0)), "numpy": np.__version__,
+            "scipy": scipy.__version__}
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kg_sample_passes_512x512_ngeo500_pmc.txt")
@@ -444,6 +453,81 @@ def bench_image(args):
                       "config": {"workload": IMAGE_WORKLOAD}, "image": f}), flush=True)
 
 
+def bench_lensing(args):
+    """The lensed-curve library (equatorial_lensing.py generate_library) on
+    the GPU: the curves mbar = 0..5 of r = r_h for 8 spins x 89 inclinations
+    (4,272 curves x 360 screen angles = 1,537,920 Brent root finds) in one
+    kgeo_rho_of_req launch, timed by HIP events; the oracle's brentq (scipy,
+    one core) on 360 of them for comparison."""
+    import torch
+    from kgeo_amd import lensing
+    spins = np.linspace(0.01, 0.99, 8)
+    incs = np.linspace(1, 89, 89)
+    S, I = np.meshgrid(spins, incs, indexing="ij")
+    sp = np.repeat(S.ravel(), 6)
+    th = np.repeat(I.ravel() * np.pi / 180, 6)
+    req = 1 + np.sqrt(1 - sp ** 2)
+    mb = np.tile(np.arange(6, dtype=np.int32), S.size)
+    vp = lensing.default_varphis()
+    torch.cuda.set_device(0)
+    for _ in range(args.warmup):
+        lensing.rho_of_req_device(sp, th, req, mb, vp)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rho, status = lensing.rho_of_req_device(sp, th, req, mb, vp)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    dt = float(np.median(ms))
+    roots = len(sp) * len(vp)
+    found = float((status == 0).float().mean())
+    cpu = None
+    try:
+        from oracle import kgeo_oracle as O
+        t0 = time.perf_counter()
+        O.rho_of_req(float(sp[6 * 40 + 1]), float(th[6 * 40 + 1]), float(req[6 * 40 + 1]), 1, vp)
+        cpu = {"roots_per_s": len(vp) / (time.perf_counter() - t0), "cores": 1, "kind": "port",
+               "sample": "one curve (360 angles) of the same library, scipy brentq"}
+    except (ImportError, ValueError, RuntimeError) as e:
+        log(f"lensing cpu sample skipped: {e}")
+    log(f"lensing: {roots} roots in {dt:.2f} ms -> {roots / (dt * 1e-3):.3e} roots/s ({found:.4f} found)")
+    print(json.dumps({"metric": "lensed-curve root finds/s", "value": roots / (dt * 1e-3), "unit": "roots/s",
+                      "ms_per_step": dt, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+                      "fraction_found": found, "cpu_baseline": cpu,
+                      "config": {"workload": "r = r_h curves, mbar 0..5, 8 spins x 89 inclinations x 360 angles"}}),
+          flush=True)
+
+
+def bench_dropin(args):
+    """The Python drop-in raytrace_ana at the raytrace workload, host arrays
+    in and out: the kernels plus the PCIe transfers of alpha, beta and the six
+    (ngeo, npix) outputs (tau, t, r, theta, phi, sigma: 6.3 GB) and the labels."""
+    import torch
+    from kgeo_amd.raytrace import raytrace_ana
+    th = TH_DEG * np.pi / 180.0
+    A, B = screen()
+    torch.cuda.set_device(0)
+    times = []
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        g = raytrace_ana(A_SPIN, [0, R_O, th, 0], [A, B], ngeo=NGEO, plotdata=False)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            times.append(dt)
+        del g
+    dt = float(np.median(times))
+    samples = len(A) * NGEO
+    log(f"drop-in raytrace_ana with host arrays: {dt * 1e3:.1f} ms -> {samples / dt:.3e} samples/s")
+    print(json.dumps({"metric": "drop-in raytrace_ana samples/s, host arrays in and out (PCIe-inclusive)",
+                      "value": samples / dt, "unit": "samples/s", "ms_per_call": dt * 1e3, "steps": args.steps,
+                      "warmup": args.warmup, "bytes_to_host": samples * 6 * 8,
+                      "config": {"workload": "raytrace_ana 512x512 ngeo=500 (a=0.94, th_o=17 deg, r_o=1e4)"}}),
+          flush=True)
+
+
 def bench_copy(args):
     """A plain device-to-device copy of known size (1 GiB read + 1 GiB
     written per launch): the reference point that scales the FETCH_SIZE and
@@ -473,7 +557,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="leave the batched-survey figures out of the line")
-    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial", "image", "copy"), default="raytrace")
+    ap.add_argument("--mode", choices=("raytrace", "batch", "equatorial", "image", "dropin", "lensing", "copy"),
+                    default="raytrace")
     ap.add_argument("--retune", action="store_true", help="batch mode: re-run the autotuner")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -489,6 +574,10 @@ def main():
         return bench_copy(args)
     if args.mode == "image":
         return bench_image(args)
+    if args.mode == "dropin":
+        return bench_dropin(args)
+    if args.mode == "lensing":
+        return bench_lensing(args)
 
     import torch
     from kgeo_amd import _lib
