@@ -25,13 +25,24 @@ How it is pinned (tests/test_oracle_*.py)
   special-case branch of ``ellip_pi`` (NaN / inf / limits).
 * Roots: residual of the radial potential R(r), Vieta's relations, and
   mpmath ``polyroots`` at 50 digits for the labels.
-* Mino time: ``mpmath.quad`` of dr/sqrt(R) for every radial class.
+* Mino time: ``mpmath.quad`` of dr/sqrt(R) for every radial class, region
+  II (rays from inside the outer horizon's turning region) included.
 * Geodesic samples: initial conditions at tau=0, the Mino-time geodesic ODE
   residual (first integrals (dr/dtau)^2=R, (dth/dtau)^2=Theta, dphi/dtau and
   dt/dtau by high-order finite differences), the equatorial consistency of
-  r_equatorial vs coords_at_tau, and the 15 ipole midplane-crossing vectors
-  that the reference's own test holds.
-* Redshift model: u.u = -1 for the Keplerian/plunging velocity.
+  r_equatorial vs coords_at_tau (also south of the equator), and the 15
+  ipole midplane-crossing vectors that the reference's own test holds.
+* Counts: equatorial crossings against sign changes of cos(theta) at every
+  sample, for observers north and south of the equator; poloidal orbits
+  against one period found by root-finding on the sampled ray; angular
+  turning points (ordinary and vortical) against the extrema of theta; the
+  radial turning count against the extrema of r.
+* Redshift model: u.u = -1 for the Keplerian/plunging velocity; g against
+  -1/(k.u) with k from finite differences of the traced ray; the
+  Schwarzschild circular-orbit closed form.
+* Lensed curves: returned radii are sign changes of r_equatorial2 - r_eq;
+  Schwarzschild curves are mirror-symmetric; critical-curve points are double
+  roots of R(r) and separate rays inside from rays outside.
 
 Parity unpinned (stated here and in the design notes): ``my_cbrt`` on an exact tie of
 real parts (the reference returns uninitialised memory there; this
