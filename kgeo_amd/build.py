@@ -3,6 +3,9 @@
   libkgeo_hip.so        HIP kernels + C-ABI for gfx950 (the product)
   libkgeo_hostcheck.so  CPU instantiation of the same device headers, used
                         only by the CPU test suite and the FLOP counter
+  libkgeo_hostcheck_san.so  the same CPU instantiation under AddressSanitizer
+                        and UndefinedBehaviorSanitizer (gcc), for the CPU
+                        suite's memory-safety run of the pass code
 
 Run:  python -m kgeo_amd.build
 """
@@ -91,8 +94,48 @@ def build_opcount():
     return out
 
 
+def sanitizer_runtime():
+    """The AddressSanitizer runtime of the compiler build_hostcheck uses
+    (clang's libclang_rt.asan, else gcc's libasan), or None."""
+    cxx = host_clang()
+    if cxx:
+        p = subprocess.run([cxx, "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True,
+                           text=True).stdout.strip()
+        if p and os.path.exists(p):
+            return cxx, p
+    p = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    if p and os.path.exists(p):
+        return "g++", p
+    return None
+
+
+def build_hostcheck_sanitized():
+    """The CPU instantiation built with AddressSanitizer and
+    UndefinedBehaviorSanitizer (host code only; nothing here is GPU code),
+    with the same compiler and floating-point contract as build_hostcheck so
+    that its outputs match it.  Skipped where no sanitizer runtime exists."""
+    rt = sanitizer_runtime()
+    if rt is None:
+        print("sanitizer runtime not found: libkgeo_hostcheck_san.so not built", flush=True)
+        return None
+    cxx = rt[0]
+    out = os.path.join(HERE, "libkgeo_hostcheck_san.so")
+    tmp = out + ".tmp"
+    flags = ["-O2", "-g", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unknown-pragmas", "-fno-omit-frame-pointer",
+             "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
+    if cxx != "g++":
+        flags += ["-shared-libsan"]
+        flags += ["-mfma", "-mavx2", "-ffp-contract=fast-honor-pragmas"] if _cpu_has_fma() else ["-ffp-contract=off"]
+    else:
+        flags += ["-ffp-contract=off"]
+    _run([cxx] + flags + ["-o", tmp, os.path.join(CSRC, "kg_hostcheck.cpp")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all():
     build_hostcheck()
+    build_hostcheck_sanitized()
     build_opcount()
     build_hip()
 

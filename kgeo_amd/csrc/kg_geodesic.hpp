@@ -1174,7 +1174,10 @@ template <class T> struct Sample {
 template <class T>
 KG_INLINE void combine(const CombineConsts<T>& k, const T& tau, const RadialOut<T>& ro, const T& th, const T& Gph,
                        const T& Gt, Sample<T>& s) {
-  T Iph = k.fac_ph * (k.cp * ro.Ip - k.cm * ro.Im);
+  // a = 0: the reference multiplies a finite bracket by exactly 0; near the
+  // horizon-end sample our bracket can round past the pole of Pi_p to inf,
+  // and 0 * inf would turn the reference's 0 into NaN
+  T Iph = (val(k.fac_ph) == 0.0) ? k.fac_ph * 0.0 : k.fac_ph * (k.cp * ro.Ip - k.cm * ro.Im);
   T ItA = k.fac_t * (k.tp * ro.Ip - k.tm * ro.Im);
   T It = ItA + 4.0 * (-tau) + 2.0 * ro.I1 + ro.I2;
   s.sig = ro.I2 + k.a2 * Gt;

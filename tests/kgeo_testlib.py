@@ -216,13 +216,14 @@ class Host:
         outs["nmax_eq"] = nm
         return outs
 
-    def raytrace_loops(self, a, r_o, th_o, alpha, beta, ngeo, tau_in=None, chunks=1, passes=3):
+    def raytrace_loops(self, a, r_o, th_o, alpha, beta, ngeo, tau_in=None, chunks=1, passes=3, do_phi_t=1):
         """The kernels' per-pixel sample code on the CPU, with ngeo split into
         `chunks` ranges as the kernels' grid does; adds the status bits.
         passes=3: the anchor / walk / fixup passes (the kernels' path on the
         linspace grid; adds the theta and radial fixup-queue lengths and the
         radial repair items as "nfix"); passes=1: one
-        combined loop per coordinate."""
+        combined loop per coordinate.  do_phi_t=0 runs the passes without t
+        and phi (NULL outputs), which only the CPU build keeps."""
         n = len(alpha)
         outs = {k: np.empty((ngeo, n)) for k in ("tau", "t", "r", "th", "ph", "sig")}
         nm = np.empty(n, np.int32)
@@ -231,7 +232,7 @@ class Host:
         ti = None if tau_in is None else self._p(np.ascontiguousarray(tau_in, float))
         self.lib.kgh_raytrace_passes(self._p(self.scalars(a, r_o, th_o)), self._p(np.ascontiguousarray(alpha, float)),
                                      self._p(np.ascontiguousarray(beta, float)), ctypes.c_int64(n),
-                                     ctypes.c_int64(ngeo), ctypes.c_int(1), ti, ctypes.c_int(chunks),
+                                     ctypes.c_int64(ngeo), ctypes.c_int(do_phi_t), ti, ctypes.c_int(chunks),
                                      ctypes.c_int(passes),
                                      *[self._p(outs[k]) for k in ("tau", "t", "r", "th", "ph", "sig")],
                                      self._p(nm, ctypes.c_int32), self._p(st, ctypes.c_uint32),
