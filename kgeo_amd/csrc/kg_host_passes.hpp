@@ -37,7 +37,7 @@ inline void unpack_host_item(uint64_t e, int64_t& i, int64_t& j, bool& walked) {
 
 // st [npix] accumulates the status bits; nfix[3] receives the theta and
 // radial queue lengths, or -1 where a pass queued more items than the
-// kernels' queue holds (npix * ceil(ngeo / kFixupSpan)), and the number of
+// kernels' queue holds (npix * fixup_items(ngeo)), and the number of
 // radial repair items.  The host queue
 // itself is unbounded, so the results stay complete either way.
 template <bool DO_PHI_T, class T>
@@ -58,7 +58,7 @@ void three_passes(const RayScalars<T>& c, const double* plan, uint32_t gflags, c
     for (int ch = 0; ch < chunks; ++ch)
       st[i] |= theta_walk_pixel<DO_PHI_T, HostPush, T>(plan, npix, i, gflags, o, i, chunk_start(G, ch, chunks),
                                                         chunk_start(G, ch + 1, chunks), rth, push);
-  const size_t cap = static_cast<size_t>(npix) * static_cast<size_t>(seg_count(G, kFixupSpan));
+  const size_t cap = static_cast<size_t>(npix) * static_cast<size_t>(fixup_items(G));
   nfix[0] = items.size() <= cap ? static_cast<int64_t>(items.size()) : -1;
   for (uint64_t e : items) {
     int64_t i, j;

@@ -425,12 +425,15 @@ KG_INLINE int64_t seg_count(int64_t ngeo, int64_t len) { return (ngeo + len - 1)
 // A queued range ends at the next multiple of kFixupSpan (or the segment's
 // end): the fixup pass runs its ranges in parallel, and a range of refused
 // steps costs a direct evaluation per sample.  Each kFixupSpan window of a
-// pixel is queued at most once, so ceil(ngeo / kFixupSpan) items per pixel
-// bound the queue.
+// pixel is queued at most once by the walk's ranges, and the last grid
+// sample at most once more on its own, so ceil(ngeo / kFixupSpan) + 1 items
+// per pixel bound the queue.
 constexpr int64_t kFixupSpan = 8;
 
+KG_INLINE int64_t fixup_items(int64_t ngeo) { return (ngeo + kFixupSpan - 1) / kFixupSpan + 1; }
+
 // Queue walked samples j .. min(segend, last) - 1 (the last grid sample is
-// the anchor pass's): from j to the next window start, then one item per
+// queued on its own): from j to the next window start, then one item per
 // window.
 template <class Push>
 KG_INLINE void push_range(Push& push, int64_t i, int64_t j, int64_t segend, int64_t last) {
@@ -687,19 +690,14 @@ KG_INLINE void radial_from_walk(const WalkStep<T, 3>& w, const WalkState<T, 3>& 
   }
 }
 
+// Radial head hd of pixel column col (direct evaluation; heads as for
+// theta).  Records 0 and 1 hold plain Jacobi functions of the step argument.
+// The head writes records only: the radial walk emits every segment start
+// from its record (radial_walk), so the anchor passes never read the theta
+// pass's G_phi, G_t and can run while the theta passes do.
 template <class T, int CASE, bool DO_PHI_T>
-KG_INLINE bool radial_emit(const SampleOut& o, const RadialConsts<T>& rc, const CombineConsts<T>& cmb, int64_t i,
-                           int64_t j, double tau, const RState<T>& rs) {
-  RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
-  return radial_store<T, DO_PHI_T>(o, cmb, j * o.npix + i, tau, ro);
-}
-
-// Radial head hd of output pixel i (direct evaluation; heads as for theta).
-// Records 0 and 1 hold plain Jacobi functions of the step argument.
-template <class T, int CASE, bool DO_PHI_T>
-KG_INLINE bool radial_head(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
-                           const CombineConsts<T>& cmb, int64_t i, int64_t hd, double stop, const Records& rt,
-                           int64_t col) {
+KG_INLINE void radial_head(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad, int64_t hd,
+                           double stop, const Records& rt, int64_t col) {
   constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
   const int64_t last = o.ngeo - 1;
   WalkStep<T, 3> w;
@@ -713,17 +711,16 @@ KG_INLINE bool radial_head(const SampleOut& o, const RadialConsts<T>& rc, const 
     radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, X, T(INFINITY), rs, st, lb);
     lb[0] = lb[1] = lb[2] = T(0.0);  // no block logs at a step
     record_store<T, 3>(rt, hd, kRFields, col, st, lb);
-    return false;
+    return;
   }
   const int64_t j = (hd == 2) ? last : (hd - 3) * kAnchorBlock;
-  if (hd > 2 && j >= last) return false;  // the last sample is head 2
+  if (hd > 2 && j >= last) return;  // the last sample is head 2
   const double tau = mino_tau(j, o.ngeo, stop, false);
   const T dv = rc.v[R_Xs] * tau;
   const T X = rc.v[R_U0] - dv;
   radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, X, dv, rs, st, lb);
   record_store<T, 3>(rt, hd == 2 ? last_record(o.ngeo, kAnchorEvery) : segment_record(j, kAnchorEvery), kRFields,
                      col, st, lb);
-  return radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, j, tau, rs);
 }
 
 // A far pass without a repair pass: a lost long step invalidates the
@@ -734,27 +731,25 @@ struct NoRepair {
   KG_INLINE void operator()(int64_t, int64_t, bool) const {}
 };
 
-// Radial segment starts of block blk after its first sample (as theta_far).
-// With a repair pass (Repair::kOn), the first lost long step hands segment
-// js and the rest of the block to it (repair(i, js, false)) instead.
+// Radial segment starts of block blk after its first sample (as theta_far),
+// records only (the walk emits the samples).  With a repair pass
+// (Repair::kOn), the first lost long step hands segment js and the rest of
+// the block to it (repair(i, js, false)) instead.
 template <class T, int CASE, bool DO_PHI_T, class Repair = NoRepair>
-KG_INLINE bool radial_far(const SampleOut& o, const RadialConsts<T>& rc, const CombineConsts<T>& cmb, int64_t i,
-                          int64_t blk, double stop, const Records& rt, int64_t col,
-                          const Repair& repair = Repair()) {
+KG_INLINE void radial_far(const SampleOut& o, const RadialConsts<T>& rc, int64_t i, int64_t blk, const Records& rt,
+                          int64_t col, const Repair& repair = Repair()) {
   constexpr int64_t L = (CASE == 2) ? kAnchorEveryLong : kAnchorEvery;
   constexpr bool kBlocks = DO_PHI_T && (CASE == 3 || CASE == 4);
-  const T m = rc.v[R_k];
   const int64_t last = o.ngeo - 1;
   const int64_t jb = blk * kAnchorBlock;
   const int64_t je = (jb + kAnchorBlock < last) ? jb + kAnchorBlock : last;
-  if (jb + L >= je) return false;
+  if (jb + L >= je) return;
   WalkStep<T, 3> w;  // the long step
   radial_init<T, CASE>(rc, w);
   record_load<T, 3>(rt, 1, kRFields, col, w.b, static_cast<T*>(nullptr));
   WalkState<T, 3> st;
   T lb[3] = {T(0.0), T(0.0), T(0.0)};
   record_load<T, 3>(rt, segment_record(jb, kAnchorEvery), kRFields, col, st, kBlocks ? lb : static_cast<T*>(nullptr));
-  bool bad = false;
   bool lost = false;
   for (int64_t js = jb + L; js < je; js += L) {
     WalkState<T, 3> nx;
@@ -763,28 +758,22 @@ KG_INLINE bool radial_far(const SampleOut& o, const RadialConsts<T>& rc, const C
     if (lost) {
       if (Repair::kOn) {
         repair(i, js, false);
-        return bad;
+        return;
       }
       record_invalidate(rt, segment_record(js, kAnchorEvery), kRFields, col);
       continue;
     }
     st = nx;
     record_store<T, 3>(rt, segment_record(js, kAnchorEvery), kRFields, col, st, lb);
-    const double tau = mino_tau(js, o.ngeo, stop, false);
-    RState<T> rs;
-    radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
-    bad |= radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, js, tau, rs);
   }
-  return bad;
 }
 
 // The repair pass: segment start js of a block whose far pass lost a long
-// step at or before js, by direct evaluation; writes its record and start
-// sample, as the far pass would have.
+// step at or before js, by direct evaluation; writes its record, as the far
+// pass would have.
 template <class T, int CASE, bool DO_PHI_T>
-KG_INLINE bool radial_repair(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
-                             const CombineConsts<T>& cmb, int64_t i, int64_t js, double stop, const Records& rt,
-                             int64_t col) {
+KG_INLINE void radial_repair(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad, int64_t js,
+                             double stop, const Records& rt, int64_t col) {
   WalkStep<T, 3> w;
   radial_init<T, CASE>(rc, w);
   const double tau = mino_tau(js, o.ngeo, stop, false);
@@ -794,7 +783,6 @@ KG_INLINE bool radial_repair(const SampleOut& o, const RadialConsts<T>& rc, cons
   T lb[3];
   radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, rc.v[R_U0] - dv, dv, rs, st, lb);
   record_store<T, 3>(rt, segment_record(js, kAnchorEvery), kRFields, col, st, lb);
-  return radial_emit<T, CASE, DO_PHI_T>(o, rc, cmb, i, js, tau, rs);
 }
 
 // Segment starts of the block of a repair item (i, js0): js0 + s * L for
@@ -824,12 +812,34 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
   bool bad = false;
   bool queued = false;
   for (int64_t j = j0; j < j1; ++j) {
-    if (j == last) continue;  // written by the anchor pass
-    if (j % L == 0) {
-      record_load<T, 3>(rt, segment_record(j, kAnchorEvery), kRFields, col, st,
-                        kBlocks ? lb : static_cast<T*>(nullptr));
-      queued = !kfinite(val(st.s));  // unreachable: the fixup pass takes the segment
-      if (queued) push_range(push, i, j, j + L, last);
+    if (j == last || j % L == 0) {
+      // a segment start or the last sample: its state is the anchor passes'
+      // record, and the sample is emitted from it here (the anchor passes
+      // write records only).  A record that is unreachable, or whose state
+      // sits too close to a singularity to be walked (non-finite entries),
+      // sends the sample, and for a segment start the segment, to the
+      // fixup pass, which evaluates them directly.
+      record_load<T, 3>(rt, j == last ? last_record(o.ngeo, kAnchorEvery) : segment_record(j, kAnchorEvery),
+                        kRFields, col, st, kBlocks ? lb : static_cast<T*>(nullptr));
+      bool ok = kfinite(val(st.s)) && kfinite(val(st.c)) && kfinite(val(st.d)) && kfinite(val(st.D));
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        ok = ok && kfinite(val(st.P[q])) && kfinite(val(st.Lg[q])) && (!kBlocks || kfinite(val(lb[q])));
+      if (!ok) {
+        if (j == last) push(i, j, false);
+        else push_range(push, i, j, j + L, last);
+        queued = j != last;
+        continue;
+      }
+      queued = false;
+      const int64_t idx = j * o.npix + i;
+      const T gph = DO_PHI_T ? T(o.ph[idx]) : T(0.0);
+      const T gt = DO_PHI_T ? T(o.t[idx]) : T(0.0);
+      const double tau = mino_tau(j, o.ngeo, stop, false);
+      RState<T> rs;
+      radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
+      RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
+      bad |= radial_store_g<T, DO_PHI_T>(o, cmb, idx, tau, ro, gph, gt);
       continue;
     }
     if (queued) continue;
@@ -977,16 +987,16 @@ KG_INLINE uint32_t radial_anchor_case(const RayScalars<T>& c, const double* __re
   T lam;
   double stop;
   radial_pick(plan, stride, p, rc, lam, stop);
-  const CombineConsts<T> cmb = combine_consts(c, lam);
+  (void)c;
   if constexpr (KIND == 1) {
-    return radial_far<T, CASE, DO_PHI_T>(o, rc, cmb, i, k, stop, rt, p, repair) ? ST_NONFINITE : 0u;
+    radial_far<T, CASE, DO_PHI_T>(o, rc, i, k, rt, p, repair);
+    return 0u;
   } else {
     JacobiLadder<T> lad;
     jacobi_ladder(rc.v[R_k], lad);
-    bool bad;
-    if constexpr (KIND == 2) bad = radial_repair<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, k, stop, rt, p);
-    else bad = radial_head<T, CASE, DO_PHI_T>(o, rc, lad, cmb, i, k, stop, rt, p);
-    return loop_status(bad, lad);
+    if constexpr (KIND == 2) radial_repair<T, CASE, DO_PHI_T>(o, rc, lad, k, stop, rt, p);
+    else radial_head<T, CASE, DO_PHI_T>(o, rc, lad, k, stop, rt, p);
+    return loop_status(false, lad);
   }
 }
 

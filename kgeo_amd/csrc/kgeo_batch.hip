@@ -337,7 +337,7 @@ int64_t theta_rec_elems(const kgeo_ray_params& p) {
 int64_t radial_rec_elems(const kgeo_ray_params& p) {
   return p.ngeo > 2 ? radial_records(p.ngeo) * kRFields * slots_of(p) : 0;
 }
-int64_t queue_elems(const kgeo_ray_params& p) { return seg_count(p.ngeo, kFixupSpan) * slots_of(p); }
+int64_t queue_elems(const kgeo_ray_params& p) { return fixup_items(p.ngeo) * slots_of(p); }
 
 struct BatchSizes {
   int64_t slots, rec, queue;

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/kgeo.h"
 #include "kg_counts.hpp"
@@ -47,6 +48,11 @@ thread_local char g_err[512] = "";
 
 int fail(int code, const char* fmt, const char* a = "", long long b = 0) {
   std::snprintf(g_err, sizeof(g_err), fmt, a, b);
+  return code;
+}
+
+int fail_msg(int code, const char* msg) {
+  std::snprintf(g_err, sizeof(g_err), "%s", msg);
   return code;
 }
 
@@ -397,9 +403,9 @@ int env_chunks() {
 // (ngeo > 2), the theta and radial record tables and the fixup queue (one
 // item per kFixupSpan samples and pixel bounds every pass's pushes).
 struct WsLayout {
-  size_t plan, threc, rrec, queue, total;
+  size_t plan, threc, rrec, queue, rqueue, total;
   int64_t nth, nr;
-  unsigned long long qcap;
+  unsigned long long qcap, rcap;
 };
 
 WsLayout ws_layout(int64_t npix, int64_t ngeo) {
@@ -414,9 +420,15 @@ WsLayout ws_layout(int64_t npix, int64_t ngeo) {
     off += static_cast<size_t>(L.nth) * kThFields * np * sizeof(double);
     L.rrec = off;
     off += static_cast<size_t>(L.nr) * kRFields * np * sizeof(double);
-    L.qcap = static_cast<unsigned long long>(np) * static_cast<unsigned long long>(seg_count(ngeo, kFixupSpan));
+    L.qcap = static_cast<unsigned long long>(np) * static_cast<unsigned long long>(fixup_items(ngeo));
     L.queue = off;
     off += static_cast<size_t>(L.qcap) * sizeof(unsigned long long);
+    // the radial far pass's repair items, at most one per (pixel, block):
+    // their own storage, as the radial anchor passes run beside the theta
+    // passes, whose fixups use the queue above
+    L.rcap = static_cast<unsigned long long>(np) * static_cast<unsigned long long>(anchor_blocks(ngeo));
+    L.rqueue = off;
+    off += static_cast<size_t>(L.rcap) * sizeof(unsigned long long);
   }
   L.total = off;
   return L;
@@ -429,21 +441,21 @@ WsLayout ws_layout(int64_t npix, int64_t ngeo) {
 // that no test has run.
 int require_phi_t(const kgeo_ray_params* p) {
   if (!(p->flags & KGEO_DO_PHI_T))
-    return fail(KGEO_E_ARG, "KGEO_DO_PHI_T is required (t, phi and sigma are always sampled)%s%lld");
+    return fail_msg(KGEO_E_ARG, "KGEO_DO_PHI_T is required (t, phi and sigma are always sampled)");
   return KGEO_OK;
 }
 
 int validate(const kgeo_ray_params* p, const double* alpha, const double* beta, void* ws, size_t wsb,
              int64_t ngeo) {
-  if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
+  if (!p) return fail_msg(KGEO_E_ARG, "params is NULL");
   if (require_phi_t(p)) return KGEO_E_ARG;
   if (p->npix < 0) return fail(KGEO_E_ARG, "npix < 0%s (%lld)", "", p->npix);
-  if (p->npix > 0 && (!alpha || !beta)) return fail(KGEO_E_ARG, "alpha/beta is NULL%s%lld");
-  if (p->npix > 0 && !ws) return fail(KGEO_E_WORKSPACE, "workspace is NULL%s%lld");
+  if (p->npix > 0 && (!alpha || !beta)) return fail_msg(KGEO_E_ARG, "alpha/beta is NULL");
+  if (p->npix > 0 && !ws) return fail_msg(KGEO_E_WORKSPACE, "workspace is NULL");
   const size_t need = kgeo_workspace_bytes(p->npix, ngeo);
   if (wsb < need)
     return fail(KGEO_E_WORKSPACE, "workspace too small%s: need %lld bytes", "", static_cast<long long>(need));
-  if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) return fail(KGEO_E_WORKSPACE, "workspace not 256-byte aligned%s%lld");
+  if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) return fail_msg(KGEO_E_WORKSPACE, "workspace not 256-byte aligned");
   return KGEO_OK;
 }
 
@@ -462,6 +474,40 @@ int run_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, 
   return check_launch("kg_setup");
 }
 
+// A second stream per device for the radial anchor passes, which depend
+// only on the plan and so run beside the theta passes (two latency-bound
+// kernels interleave on the SIMDs); created once per device, with the fork
+// and join events.  The fork, the side stream's work and the join are
+// enqueued under one lock, so calls from several host threads keep their
+// own dependencies (their radial anchor passes take turns on the side
+// stream).  Streams keep the default priority.
+struct SideStream {
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool tried = false;
+};
+constexpr int kMaxDev = 64;
+std::mutex g_side_mu;
+SideStream g_side[kMaxDev];
+
+// The device's side stream, or nullptr (the passes then run in order on the
+// caller's stream).  Call with g_side_mu held.
+SideStream* side_stream() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+  SideStream& s = g_side[dev];
+  if (!s.tried) {
+    s.tried = true;
+    if (hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      s.side = nullptr;
+    }
+  }
+  return s.side ? &s : nullptr;
+}
+
 template <bool DO_PHI_T>
 int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipStream_t st) {
   const WsLayout L = ws_layout(d.npix, d.ngeo);
@@ -470,32 +516,51 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   const Records rth{reinterpret_cast<double*>(base + L.threc), d.npix, 0};
   const Records rr{reinterpret_cast<double*>(base + L.rrec), d.npix, 0};
   unsigned long long* items = reinterpret_cast<unsigned long long*>(base + L.queue);
+  unsigned long long* ritems = reinterpret_cast<unsigned long long*>(base + L.rqueue);
   const FixupQueue qth{items, flags + kThetaCount, L.qcap};
   const FixupQueue qr{items, flags + kRadialCount, L.qcap};
-  const FixupQueue qrep{items, flags + kRepairCount, L.qcap};
+  const FixupQueue qrep{ritems, flags + kRepairCount, L.rcap};
   const unsigned nb = static_cast<unsigned>((d.npix + kPassBlock - 1) / kPassBlock);
   const dim3 wgrid(nb, grid.y);
   const unsigned long long fb = (L.qcap + kPassBlock - 1) / kPassBlock;
   const dim3 fgrid(static_cast<unsigned>(fb < 4096ull ? fb : 4096ull));
+  const unsigned long long rb = (L.rcap * kRepairSegments + kPassBlock - 1) / kPassBlock;
+  const dim3 rgrid(static_cast<unsigned>(rb < 4096ull ? (rb > 0 ? rb : 1ull) : 4096ull));
   auto recs = [](int64_t n) { return static_cast<unsigned>(n < 65535 ? n : 65535); };
   hipLaunchKernelGGL(kg_clear_counters, dim3(1), dim3(64), 0, st, flags);
   int rc = check_launch("kg_clear_counters");
   if (rc) return rc;
   const int64_t nblk = anchor_blocks(d.ngeo), nhead = anchor_heads(d.ngeo);
-  hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rth, nhead);
-  if ((rc = check_launch("kg_theta_heads"))) return rc;
-  hipLaunchKernelGGL(kg_theta_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rth, nblk);
-  if ((rc = check_launch("kg_theta_far"))) return rc;
-  hipLaunchKernelGGL(kg_theta_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rth, qth);
-  if ((rc = check_launch("kg_theta_walk"))) return rc;
-  hipLaunchKernelGGL(kg_theta_fixup<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, qth);
-  if ((rc = check_launch("kg_theta_fixup"))) return rc;
-  hipLaunchKernelGGL(kg_radial_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rr, nhead);
-  if ((rc = check_launch("kg_radial_heads"))) return rc;
-  hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rr, nblk, qrep);
-  if ((rc = check_launch("kg_radial_far"))) return rc;
-  hipLaunchKernelGGL(kg_radial_repair<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, rr, qrep);
-  if ((rc = check_launch("kg_radial_repair"))) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    SideStream* ss = side_stream();
+    const hipStream_t rs = ss ? ss->side : st;
+    if (ss) {
+      if (hipEventRecord(ss->fork, st) != hipSuccess || hipStreamWaitEvent(rs, ss->fork, 0) != hipSuccess)
+        return fail_msg(KGEO_E_HIP, "fork to the side stream failed");
+    }
+    // radial heads, far and repair: records only (the walk emits the samples)
+    hipLaunchKernelGGL(kg_radial_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, rs, d, a, rr, nhead);
+    if ((rc = check_launch("kg_radial_heads"))) return rc;
+    hipLaunchKernelGGL(kg_radial_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, rs, d, a, rr, nblk,
+                       qrep);
+    if ((rc = check_launch("kg_radial_far"))) return rc;
+    hipLaunchKernelGGL(kg_radial_repair<DO_PHI_T>, rgrid, dim3(kPassBlock), 0, rs, d, a, rr, qrep);
+    if ((rc = check_launch("kg_radial_repair"))) return rc;
+    if (ss && hipEventRecord(ss->join, rs) != hipSuccess) return fail_msg(KGEO_E_HIP, "join event failed");
+    // theta passes on the caller's stream meanwhile
+    hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rth, nhead);
+    if ((rc = check_launch("kg_theta_heads"))) return rc;
+    hipLaunchKernelGGL(kg_theta_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rth, nblk);
+    if ((rc = check_launch("kg_theta_far"))) return rc;
+    hipLaunchKernelGGL(kg_theta_walk<DO_PHI_T>, grid, dim3(kBlock), 0, st, d, a, rth, qth);
+    if ((rc = check_launch("kg_theta_walk"))) return rc;
+    hipLaunchKernelGGL(kg_theta_fixup<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, qth);
+    if ((rc = check_launch("kg_theta_fixup"))) return rc;
+    if (ss && hipStreamWaitEvent(st, ss->join, 0) != hipSuccess) return fail_msg(KGEO_E_HIP, "join failed");
+  }
+  // the radial walk emits every radial sample it walks and every segment
+  // start from its record, combining with the theta pass's G_phi, G_t
   hipLaunchKernelGGL(kg_radial_walk<DO_PHI_T>, wgrid, dim3(kPassBlock), 0, st, d, a, rr, qr);
   if ((rc = check_launch("kg_radial_walk"))) return rc;
   hipLaunchKernelGGL(kg_radial_fixup<DO_PHI_T>, fgrid, dim3(kPassBlock), 0, st, d, a, qr);
@@ -506,7 +571,7 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
                double* th, double* ph, double* sig, uint32_t* status, void* ws, hipStream_t st) {
   DevParams d = make_dev(p, ngeo);
   if (d.npix == 0 || ngeo <= 0) return KGEO_OK;
-  if (!t || !ph) return fail(KGEO_E_ARG, "t and ph outputs are required%s%lld");
+  if (!t || !ph) return fail_msg(KGEO_E_ARG, "t and ph outputs are required");
   const uint32_t* gflags = static_cast<const uint32_t*>(ws);
   const double* plan = reinterpret_cast<const double*>(static_cast<const char*>(ws) + kFlagBytes);
   const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
@@ -567,10 +632,10 @@ int kgeo_plan(const kgeo_ray_params* p, const double* alpha, const double* beta,
 int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, double* tau, double* t, double* r,
                 double* th, double* ph, double* sig, uint32_t* status, void* workspace, size_t workspace_bytes,
                 void* stream) {
-  if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
+  if (!p) return fail_msg(KGEO_E_ARG, "params is NULL");
   if (require_phi_t(p)) return KGEO_E_ARG;
-  if (p->npix < 0 || ntau < 0) return fail(KGEO_E_ARG, "negative size%s%lld");
-  if (p->npix > 0 && !workspace) return fail(KGEO_E_WORKSPACE, "workspace is NULL%s%lld");
+  if (p->npix < 0 || ntau < 0) return fail_msg(KGEO_E_ARG, "negative size");
+  if (p->npix > 0 && !workspace) return fail_msg(KGEO_E_WORKSPACE, "workspace is NULL");
   const size_t need = kgeo_workspace_bytes(p->npix, tau_in ? 0 : ntau);
   if (workspace_bytes < need)
     return fail(KGEO_E_WORKSPACE, "workspace too small%s: need %lld bytes", "", static_cast<long long>(need));
@@ -596,7 +661,7 @@ int kgeo_coords_at_tau(const kgeo_ray_params* p, const double* alpha, const doub
   int rc = validate(p, alpha, beta, workspace, workspace_bytes, 0);
   if (rc) return rc;
   if (ntau < 0) return fail(KGEO_E_ARG, "ntau < 0%s (%lld)", "", ntau);
-  if (ntau > 0 && p->npix > 0 && !tau_in) return fail(KGEO_E_ARG, "tau_in is NULL%s%lld");
+  if (ntau > 0 && p->npix > 0 && !tau_in) return fail_msg(KGEO_E_ARG, "tau_in is NULL");
   hipStream_t st = static_cast<hipStream_t>(stream);
   rc = run_plan(p, alpha, beta, nullptr, nullptr, nullptr, nullptr, status, workspace, st);
   if (rc) return rc;
@@ -605,10 +670,10 @@ int kgeo_coords_at_tau(const kgeo_ray_params* p, const double* alpha, const doub
 
 int kgeo_crossings(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau,
                    int64_t ntau, double* n_equatorial, double* n_poloidal, void* stream) {
-  if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
-  if (p->npix < 0 || ntau < 0) return fail(KGEO_E_ARG, "negative size%s%lld");
+  if (!p) return fail_msg(KGEO_E_ARG, "params is NULL");
+  if (p->npix < 0 || ntau < 0) return fail_msg(KGEO_E_ARG, "negative size");
   if (p->npix == 0 || ntau == 0) return KGEO_OK;
-  if (!alpha || !beta || !tau) return fail(KGEO_E_ARG, "NULL array%s%lld");
+  if (!alpha || !beta || !tau) return fail_msg(KGEO_E_ARG, "NULL array");
   DevParams d = make_dev(p, ntau);
   const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(kg_counts, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), d, alpha, beta, tau,
@@ -618,13 +683,13 @@ int kgeo_crossings(const kgeo_ray_params* p, const double* alpha, const double* 
 
 int kgeo_turnings(const kgeo_ray_params* p, const double* alpha, const double* beta, const double* tau,
                   int64_t ntau, double* n_angular, int32_t* turns_theta, int8_t* turns_r, void* stream) {
-  if (!p) return fail(KGEO_E_ARG, "params is NULL%s%lld");
-  if (p->npix < 0 || ntau < 0) return fail(KGEO_E_ARG, "negative size%s%lld");
+  if (!p) return fail_msg(KGEO_E_ARG, "params is NULL");
+  if (p->npix < 0 || ntau < 0) return fail_msg(KGEO_E_ARG, "negative size");
   // n_angular_turnings accepts th_o in [0, pi/2] only (kerr_raytracing_utils.py:690-691)
   if ((n_angular || turns_theta) && !(p->th_o >= 0.0 && p->th_o <= kHalfPi))
-    return fail(KGEO_E_ARG, "th_o should be a float in range (0,pi/2]%s%lld");
+    return fail_msg(KGEO_E_ARG, "th_o should be a float in range (0,pi/2]");
   if (p->npix == 0 || ntau == 0) return KGEO_OK;
-  if (!alpha || !beta || !tau) return fail(KGEO_E_ARG, "NULL array%s%lld");
+  if (!alpha || !beta || !tau) return fail_msg(KGEO_E_ARG, "NULL array");
   DevParams d = make_dev(p, ntau);
   const unsigned nb = static_cast<unsigned>((d.npix + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(kg_turnings, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), d, alpha, beta, tau,
@@ -635,7 +700,7 @@ int kgeo_turnings(const kgeo_ray_params* p, const double* alpha, const double* b
 int kgeo_ellip_pi(const double* n, const double* phi, const double* m, double* out, int64_t count, void* stream) {
   if (count < 0) return fail(KGEO_E_ARG, "count < 0%s (%lld)", "", count);
   if (count == 0) return KGEO_OK;
-  if (!n || !phi || !m || !out) return fail(KGEO_E_ARG, "NULL array%s%lld");
+  if (!n || !phi || !m || !out) return fail_msg(KGEO_E_ARG, "NULL array");
   const unsigned nb = static_cast<unsigned>((count + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(kg_ellip_pi, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, phi, m, out, count);
   return check_launch("kg_ellip_pi");

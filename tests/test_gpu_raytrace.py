@@ -187,6 +187,39 @@ def test_other_regimes(case):
     _check(got, ref, a, 1e4, th, A, B, f"a={a} th={th_deg} n={n} G={G}", gate_p99=True, ill_bound=1e-2)
 
 
+def test_zero_spin_near_axis_pixels():
+    """a = 0 with screen points on and next to the axes (impact parameter
+    down to 1e-9: radial infall) among ordinary pixels.  The reference's
+    I_phi is exactly 0 at a = 0, also at the horizon-end sample where the phi
+    bracket is largest: phi and every NaN pattern as the oracle's.  The
+    near-axis pixels have degenerate radial roots (r = 0 and three roots of
+    size (2 eta)^(1/3) ~ 1e-6), which condition tau_tot, and through the grid
+    r, t and sigma, at 1e-9 in the reference's own arithmetic: they are held
+    to that, the ordinary pixels to the full bar."""
+    rng = np.random.default_rng(7)
+    A = np.concatenate([[1e-9, 0.0, 3.0, -1e-6, 0.0], rng.uniform(-8, 8, 59)])
+    B = np.concatenate([[0.0, 5.0, 0.0, 1e-7, 1e-9], rng.uniform(-8, 8, 59)])
+    axis = np.zeros(len(A), bool)
+    axis[[0, 3, 4]] = True
+    for th_deg in (60.0, 120.0):
+        th = np.deg2rad(th_deg)
+        for G in (3, 50):
+            ref = O.raytrace_ana(0.0, 1e4, th, A, B, G)
+            got = _gpu(0.0, 1e4, th, A, B, G)
+            label = f"a=0 th={th_deg} G={G}"
+            for k in ("r", "th", "ph", "t", "sig"):
+                assert np.array_equal(np.isnan(got[k]), np.isnan(ref[k])), (label, k)
+            e_ph = np.nan_to_num(errors(got["ph"][:, axis], ref["ph"][:, axis], "ph"))
+            assert e_ph.max() < 1e-10, (label, float(e_ph.max()))
+            assert np.max(np.abs(got["tau"][:, axis] - ref["tau"][:, axis]) / np.maximum(ref["tau"][:, axis],
+                                                                                          1e-300)) < 1e-8
+            def sub(d):
+                return {k: (tuple(x[..., ~axis] for x in v) if isinstance(v, tuple) else
+                            v[..., ~axis] if isinstance(v, np.ndarray) and v.shape[-1] == len(A) else v)
+                        for k, v in d.items()}
+            _check(sub(got), sub(ref), 0.0, 1e4, th, A[~axis], B[~axis], label, gate_p99=True, ill_bound=1e-2)
+
+
 def test_single_pixel_and_ngeo_one():
     a, th = 0.5, 0.7
     ref = O.raytrace_ana(a, 1e3, th, np.array([2.5]), np.array([-1.5]), 1)
