@@ -425,15 +425,14 @@ KG_INLINE int64_t seg_count(int64_t ngeo, int64_t len) { return (ngeo + len - 1)
 // A queued range ends at the next multiple of kFixupSpan (or the segment's
 // end): the fixup pass runs its ranges in parallel, and a range of refused
 // steps costs a direct evaluation per sample.  Each kFixupSpan window of a
-// pixel is queued at most once by the walk's ranges, and the last grid
-// sample at most once more on its own, so ceil(ngeo / kFixupSpan) + 1 items
-// per pixel bound the queue.
+// pixel is queued at most once, so ceil(ngeo / kFixupSpan) items per pixel
+// bound the queue.
 constexpr int64_t kFixupSpan = 8;
 
-KG_INLINE int64_t fixup_items(int64_t ngeo) { return (ngeo + kFixupSpan - 1) / kFixupSpan + 1; }
+KG_INLINE int64_t fixup_items(int64_t ngeo) { return (ngeo + kFixupSpan - 1) / kFixupSpan; }
 
 // Queue walked samples j .. min(segend, last) - 1 (the last grid sample is
-// queued on its own): from j to the next window start, then one item per
+// the anchor pass's): from j to the next window start, then one item per
 // window.
 template <class Push>
 KG_INLINE void push_range(Push& push, int64_t i, int64_t j, int64_t segend, int64_t last) {
@@ -483,6 +482,29 @@ KG_INLINE void record_load(const Records& rt, int64_t rec, int nf, int64_t col, 
     st.Lg[q] = T(*rt.at(rec, nf, 4 + NP + q, col));
     if (lb) lb[q] = T(*rt.at(rec, nf, 4 + 2 * NP + q, col));
   }
+}
+
+// The last grid sample's radial record: r, I_+, I_-, I_1, I_2 of its direct
+// evaluation (fields 0..4), which the walk combines with the theta pass's
+// G_phi, G_t.
+template <class T>
+KG_INLINE void last_out_store(const Records& rt, int64_t rec, int64_t col, const RadialOut<T>& ro) {
+  *rt.at(rec, kRFields, 0, col) = val(ro.r);
+  *rt.at(rec, kRFields, 1, col) = val(ro.Ip);
+  *rt.at(rec, kRFields, 2, col) = val(ro.Im);
+  *rt.at(rec, kRFields, 3, col) = val(ro.I1);
+  *rt.at(rec, kRFields, 4, col) = val(ro.I2);
+}
+
+template <class T>
+KG_INLINE RadialOut<T> last_out_load(const Records& rt, int64_t rec, int64_t col) {
+  RadialOut<T> ro;
+  ro.r = T(*rt.at(rec, kRFields, 0, col));
+  ro.Ip = T(*rt.at(rec, kRFields, 1, col));
+  ro.Im = T(*rt.at(rec, kRFields, 2, col));
+  ro.I1 = T(*rt.at(rec, kRFields, 3, col));
+  ro.I2 = T(*rt.at(rec, kRFields, 4, col));
+  return ro;
 }
 
 // Record of the segment starting at sample js (segments of `len` samples in
@@ -719,8 +741,13 @@ KG_INLINE void radial_head(const SampleOut& o, const RadialConsts<T>& rc, const 
   const T dv = rc.v[R_Xs] * tau;
   const T X = rc.v[R_U0] - dv;
   radial_direct<T, CASE, DO_PHI_T>(rc, lad, w, X, dv, rs, st, lb);
-  record_store<T, 3>(rt, hd == 2 ? last_record(o.ngeo, kAnchorEvery) : segment_record(j, kAnchorEvery), kRFields,
-                     col, st, lb);
+  if (hd == 2) {
+    // no step starts from the last sample: its record holds the sample's
+    // radial outputs, from this direct evaluation, for the walk to emit
+    last_out_store(rt, last_record(o.ngeo, kAnchorEvery), col, radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T));
+    return;
+  }
+  record_store<T, 3>(rt, segment_record(j, kAnchorEvery), kRFields, col, st, lb);
 }
 
 // A far pass without a repair pass: a lost long step invalidates the
@@ -811,34 +838,32 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
   T lb[3] = {T(0.0), T(0.0), T(0.0)};
   bool bad = false;
   bool queued = false;
-  for (int64_t j = j0; j < j1; ++j) {
-    if (j == last || j % L == 0) {
-      // a segment start or the last sample: its state is the anchor passes'
-      // record, and the sample is emitted from it here (the anchor passes
-      // write records only).  A record that is unreachable, or whose state
-      // sits too close to a singularity to be walked (non-finite entries),
-      // sends the sample, and for a segment start the segment, to the
-      // fixup pass, which evaluates them directly.
-      record_load<T, 3>(rt, j == last ? last_record(o.ngeo, kAnchorEvery) : segment_record(j, kAnchorEvery),
-                        kRFields, col, st, kBlocks ? lb : static_cast<T*>(nullptr));
+  const int64_t jw = j1 < last ? j1 : last;
+  for (int64_t j = j0; j < jw; ++j) {
+    if (j % L == 0) {
+      // a segment start: its state is the anchor passes' record, and the
+      // sample is emitted from it here (the anchor passes write records
+      // only).  A record that is unreachable, or whose state sits too close
+      // to a singularity to be walked (non-finite entries), sends the
+      // segment to the fixup pass, which evaluates it directly.
+      record_load<T, 3>(rt, segment_record(j, kAnchorEvery), kRFields, col, st,
+                        kBlocks ? lb : static_cast<T*>(nullptr));
       bool ok = kfinite(val(st.s)) && kfinite(val(st.c)) && kfinite(val(st.d)) && kfinite(val(st.D));
 #pragma unroll
       for (int q = 0; q < 3; ++q)
         ok = ok && kfinite(val(st.P[q])) && kfinite(val(st.Lg[q])) && (!kBlocks || kfinite(val(lb[q])));
-      if (!ok) {
-        if (j == last) push(i, j, false);
-        else push_range(push, i, j, j + L, last);
-        queued = j != last;
+      queued = !ok;
+      if (queued) {
+        push_range(push, i, j, j + L, last);
         continue;
       }
-      queued = false;
       const int64_t idx = j * o.npix + i;
       const T gph = DO_PHI_T ? T(o.ph[idx]) : T(0.0);
       const T gt = DO_PHI_T ? T(o.t[idx]) : T(0.0);
       const double tau = mino_tau(j, o.ngeo, stop, false);
       RState<T> rs;
       radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
-      RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
+      const RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
       bad |= radial_store_g<T, DO_PHI_T>(o, cmb, idx, tau, ro, gph, gt);
       continue;
     }
@@ -861,6 +886,14 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
     radial_from_walk(w, st, lb, rc.v[R_U0] - rc.v[R_Xs] * tau, m, DO_PHI_T, rs);
     RadialOut<T> ro = radial_out<T, CASE>(rc, T(tau), rs, DO_PHI_T);
     bad |= radial_store_g<T, DO_PHI_T>(o, cmb, idx, tau, ro, gph, gt);
+  }
+  if (j1 > last && j0 <= last) {
+    // the last sample: the head's direct outputs, kept in its record
+    const int64_t idx = last * o.npix + i;
+    const T gph = DO_PHI_T ? T(o.ph[idx]) : T(0.0);
+    const T gt = DO_PHI_T ? T(o.t[idx]) : T(0.0);
+    const RadialOut<T> ro = last_out_load<T>(rt, last_record(o.ngeo, kAnchorEvery), col);
+    bad |= radial_store_g<T, DO_PHI_T>(o, cmb, idx, mino_tau(last, o.ngeo, stop, false), ro, gph, gt);
   }
   return bad;
 }

@@ -47,9 +47,9 @@ def test_layout_offsets_and_workspace():
     slots = sum((n + 63) // 64 * 64 for n in (1, 63, 64, 65, 1000))
     plan_slots = (_lib.workspace_bytes(1000) - 256) // (1000 * 8)
     # per pixel slot at ngeo = 10: the plan, theta records (ceil(10/32) + 3 of 6
-    # fields), radial records (ceil(10/16) + 3 of 13 fields), three queue items
-    # (the ceil(10/8) ranges and the last sample)
-    core = slots * 8 * (plan_slots + 4 * 6 + 4 * 13 + 3)
+    # fields), radial records (ceil(10/16) + 3 of 13 fields), two queue items
+    # (the ceil(10/8) ranges)
+    core = slots * 8 * (plan_slots + 4 * 6 + 4 * 13 + 2)
     ws = lib.kgeo_batch_workspace_bytes(params, len(ragged))
     assert ws >= core and ws - core < 4096
 

@@ -43,13 +43,13 @@ def test_workspace_holds_the_walk_tables():
     """On a linspace grid of ngeo > 2 samples the workspace adds the record
     tables of the anchor pass (theta: ceil(ngeo/32) + 3 records of 6 fields;
     radial: ceil(ngeo/16) + 3 records of 13 fields), the fixup queue (one
-    8-byte item per 8 samples, and one for the last sample) and the repair
-    queue (one item per 256-sample block), per pixel."""
+    8-byte item per 8 samples) and the repair queue (one item per 256-sample
+    block), per pixel."""
     n = 1000
     plan = _lib.workspace_bytes(n)
     assert _lib.workspace_bytes(n, 2) == plan
-    assert _lib.workspace_bytes(n, 500) == plan + n * 8 * (19 * 6 + 35 * 13 + 64 + 2)
-    assert _lib.workspace_bytes(n, 33) == plan + n * 8 * (5 * 6 + 6 * 13 + 6 + 1)
+    assert _lib.workspace_bytes(n, 500) == plan + n * 8 * (19 * 6 + 35 * 13 + 63 + 2)
+    assert _lib.workspace_bytes(n, 33) == plan + n * 8 * (5 * 6 + 6 * 13 + 5 + 1)
 
 
 def _plan_slots():
