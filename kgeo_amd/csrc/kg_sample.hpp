@@ -209,14 +209,14 @@ KG_INLINE bool radial_block_step(const RadialConsts<T>& rc, const WalkState<T, 3
   for (int q = 0; q < 3; ++q) {
     y[q] = walk_log_y(a0[q], b0[q], a1[q], b1[q], iden[q]);
     const double y2 = kfabs(val(y[q] * y[q]));
-    if (y2 <= kStepZ2) ymax = kmax(ymax, y2);
+    ymax = kmax(ymax, y2 <= kStepZ2 ? y2 : 0.0);
   }
   const int nt = rc_terms<ADAPT>(ymax);
   bool ok = true;
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     lb[q] = walk_log(lb[q], y[q], a1[q], b1[q], nt);
-    ok = ok && kfabs(val(lb[q])) <= kLogGuard;
+    ok = ok & (kfabs(val(lb[q])) <= kLogGuard);
   }
   return ok;
 }

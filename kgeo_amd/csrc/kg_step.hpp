@@ -195,8 +195,9 @@ template <class T> KG_INLINE T walk_log_y(const T& a0, const T& b0, const T& a1,
 template <class T>
 KG_INLINE T walk_log(const T& Lg0, const T& y, const T& a1, const T& b1, int nt) {
   const T y2 = y * y;
-  if (kfabs(val(y2)) <= kStepZ2) return Lg0 + 2.0 * y * rc_one_series(y2, nt);
-  return log_ratio(a1, b1);
+  T lg = Lg0 + 2.0 * y * rc_one_series(y2, nt);
+  if (!(kfabs(val(y2)) <= kStepZ2)) lg = log_ratio(a1, b1);
+  return lg;
 }
 
 template <class T> KG_INLINE T walk_log_den(const T& a0, const T& b0, const T& a1, const T& b1) {
@@ -228,7 +229,7 @@ KG_INLINE bool walk_jacobi(const WalkState<T, NP>& a, const WalkStep<T, NP>& w, 
   out.s = S;
   out.c = C;
   out.d = Dn;
-  return kfinite(val(S)) && kfinite(val(C)) && kfinite(val(Dn));
+  return kfinite(val(S)) & kfinite(val(C)) & kfinite(val(Dn));
 }
 
 // The rest of the step a -> a + du, given out.s, out.c, out.d from
@@ -258,7 +259,8 @@ KG_INLINE bool walk_integrals(const WalkState<T, NP>& a, const WalkStep<T, NP>& 
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     z2v[i] = w.w2[i] * (q * ipv[i]) * (q * ipv[i]);
-    if (kfabs(val(z2v[i])) <= kStepZ2) zmax = kmax(zmax, kfabs(val(z2v[i])));  // others are refused
+    const double az = kfabs(val(z2v[i]));
+    zmax = kmax(zmax, az <= kStepZ2 ? az : 0.0);  // others are refused
   }
   const int nt = rc_terms<ADAPT>(zmax);
 #pragma unroll
@@ -276,15 +278,15 @@ KG_INLINE bool walk_integrals(const WalkState<T, NP>& a, const WalkStep<T, NP>& 
     // next to the integrand's pole (1 - n sn^2 -> 0: n > 1, or n -> 1 at
     // sn -> 1) 1 - n sn^2 carries an absolute rounding error of a few ulp,
     // which each step would add to Pi relative to 1 / (1 - n sn^2)
-    ok = ok && kfabs(val(r1)) >= kPoleGuard;
     const bool circ = FAR && val(w.w2[i]) < 0.0 && !(series && psign);
-    ok = ok && ((val(r0) > 0.0) == (val(r1) > 0.0)) &&
-         (circ || (psign && (series || (FAR && val(z2) <= kFarZ2))));
+    // bitwise: every test is evaluated, with no branch per test
+    ok = ok & (kfabs(val(r1)) >= kPoleGuard) & ((val(r0) > 0.0) == (val(r1) > 0.0)) &
+         (circ | (psign & (series | (FAR && val(z2) <= kFarZ2))));
     T corr;
     if (circ) corr = w.ncw[i] * katan2(w.cw[i] * q, p);
     else corr = nq * ((!FAR || series) ? rc_one_series(z2, nt) : rc_1pe(-z2));
     out.P[i] = a.P[i] + b.P[i] + corr;
-    ok = ok && kfinite(val(out.P[i]));
+    ok = ok & kfinite(val(out.P[i]));
   }
   // the transformed integrals' log terms, one shared reciprocal
   T yd[NP], iyd[NP];
@@ -298,7 +300,8 @@ KG_INLINE bool walk_integrals(const WalkState<T, NP>& a, const WalkStep<T, NP>& 
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     yv[i] = w.pt[i] * num * iyd[i];
-    if (w.tr[i] && kfabs(val(yv[i] * yv[i])) <= kStepZ2) ymax = kmax(ymax, kfabs(val(yv[i] * yv[i])));
+    const double ay = kfabs(val(yv[i] * yv[i]));
+    ymax = kmax(ymax, (w.tr[i] & (ay <= kStepZ2)) ? ay : 0.0);
   }
   const int nty = rc_terms<ADAPT>(ymax);
 #pragma unroll
@@ -306,17 +309,15 @@ KG_INLINE bool walk_integrals(const WalkState<T, NP>& a, const WalkStep<T, NP>& 
     if (w.tr[i]) {
       const T& y = yv[i];
       const T y2 = y * y;
-      if (kfabs(val(y2)) <= kStepZ2) {
-        out.Lg[i] = a.Lg[i] + 2.0 * y * rc_one_series(y2, nty);
-      } else {
-        out.Lg[i] = log_ratio(w.pt[i] * S, C * Dn);
-      }
-      ok = ok && kfabs(val(out.Lg[i])) <= kLogGuard;  // false for NaN
+      T lg = a.Lg[i] + 2.0 * y * rc_one_series(y2, nty);
+      if (!(kfabs(val(y2)) <= kStepZ2)) lg = log_ratio(w.pt[i] * S, C * Dn);
+      out.Lg[i] = lg;
+      ok = ok & (kfabs(val(lg)) <= kLogGuard);  // false for NaN
     } else {
       out.Lg[i] = T(0.0);
     }
   }
-  return ok && kfinite(val(out.D));
+  return ok & kfinite(val(out.D));
 }
 
 // One step a -> a + du.  Returns false (leaving `out` unspecified) when the
