@@ -131,6 +131,49 @@ def test_full_size_invariants_and_determinism():
     torch.cuda.empty_cache()
 
 
+def test_raytrace_replays_under_graph_capture():
+    """kgeo_raytrace_ana captured into a HIP graph, with its side-stream fork
+    and join, replays to the eager result bit for bit: three replays over
+    outputs reset to NaN, with eager work and synchronisation in between."""
+    import torch
+    from kgeo_amd.raytrace import raytrace_ana_device
+    a, th, ngeo = 0.94, np.deg2rad(17.0), 100
+    A, B = grid(8.0, 64)
+    dev = torch.device("cuda", 0)
+    al = torch.tensor(A, device=dev)
+    be = torch.tensor(B, device=dev)
+    keys = ("tau", "t", "r", "th", "ph", "sig")
+    eager = raytrace_ana_device(a, 1e4, th, al, be, ngeo)
+    ref = {k: getattr(eager, k).clone() for k in keys}
+    out = {k: torch.empty_like(ref[k]) for k in keys}
+    for k in ("rclass", "thclass", "nmax_eq", "tau_tot", "status", "workspace"):
+        out[k] = getattr(eager, k)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        raytrace_ana_device(a, 1e4, th, al, be, ngeo, out=out)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        raytrace_ana_device(a, 1e4, th, al, be, ngeo, out=out)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for k in keys:
+            out[k].fill_(float("nan"))
+        junk = torch.randn(1 << 20, device=dev, dtype=torch.float64)
+        torch.cuda.synchronize()
+        g.replay()
+        junk.mul_(2.0)
+        torch.cuda.synchronize()
+        for k in keys:
+            nan = torch.isnan(ref[k])
+            assert torch.equal(torch.isnan(out[k]), nan), (rep, k)
+            assert torch.equal(out[k][~nan], ref[k][~nan]), (rep, k)
+    del g, out, ref, eager
+    torch.cuda.empty_cache()
+
+
 def test_walk_matches_direct_evaluation():
     """The walking passes (heads, far, walk, fixup) against direct
     evaluation at the same Mino times (kgeo_coords_at_tau: Landen ladder and
