@@ -563,8 +563,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (every box has one MI355X); "
-              "the raytracer shards over pixels with no collectives, see DESIGN.md", file=sys.stderr)
+        print("bench.py: measures one GPU only (--gpus 1); a multi-GPU run would shard the pixels, "
+              "with no collectives on the path, and is not built", file=sys.stderr)
         sys.exit(2)
     if args.mode == "batch":
         return bench_batch(args)
