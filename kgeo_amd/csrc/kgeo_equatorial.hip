@@ -30,6 +30,10 @@ namespace {
 thread_local char g_err_eq[512] = "";
 
 constexpr int kBlock = 256;
+// one wave per workgroup for the kernels whose lanes differ in cost (root
+// finds of varying length, subrings that miss the disk): a CU frees a
+// workgroup's slots only when its slowest wave ends
+constexpr int kLaneBlock = 64;
 
 struct EqParams {
   RayScalars<double> c;
@@ -50,9 +54,9 @@ struct EqOut {
   uint8_t* zero_mask;
 };
 
-__global__ __launch_bounds__(kBlock) void kg_eq_image(EqParams p, const double* __restrict__ alpha,
+__global__ __launch_bounds__(kLaneBlock) void kg_eq_image(EqParams p, const double* __restrict__ alpha,
                                                       const double* __restrict__ beta, EqOut o) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kLaneBlock + threadIdx.x;
   if (i >= p.npix) return;
   EqPixel<double> e;
   equatorial_pixel(p.c, alpha[i], beta[i], p.p_emis, p.ri, p.mbar_max, e);
@@ -152,12 +156,12 @@ __global__ __launch_bounds__(kBlock) void kg_eq_reduce(const double* __restrict_
 
 // One lane per (curve k, angle i): rho_of_req_single (equatorial_lensing.py:249-262)
 // for curve k's spin, inclination, target radius and subring at varphi[i].
-__global__ __launch_bounds__(kBlock) void kg_rho_of_req(const kgeo_ray_params* __restrict__ curves,
+__global__ __launch_bounds__(kLaneBlock) void kg_rho_of_req(const kgeo_ray_params* __restrict__ curves,
                                                         const double* __restrict__ req,
                                                         const int32_t* __restrict__ mbar, int64_t ncurves,
                                                         const double* __restrict__ varphi, int64_t nphi,
                                                         double* __restrict__ rho, int32_t* __restrict__ status) {
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kLaneBlock + threadIdx.x;
   if (g >= ncurves * nphi) return;
   const int64_t k = g / nphi, i = g - (g / nphi) * nphi;
   const kgeo_ray_params& p = curves[k];
@@ -292,8 +296,8 @@ int kgeo_equatorial_image(const kgeo_ray_params* p, const double* alpha, const d
   ep.npix = p->npix;
   ep.mbar_max = mbar_max;
   EqOut o{image, I_sub, g_sub, r_s_sub, Ir_sub, Imax, Nmax, zero_mask};
-  const unsigned nb = static_cast<unsigned>((p->npix + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(kg_eq_image, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), ep, alpha, beta, o);
+  const unsigned nb = static_cast<unsigned>((p->npix + kLaneBlock - 1) / kLaneBlock);
+  hipLaunchKernelGGL(kg_eq_image, dim3(nb), dim3(kLaneBlock), 0, static_cast<hipStream_t>(stream), ep, alpha, beta, o);
   return eq_launch("kg_eq_image");
 }
 
@@ -304,8 +308,8 @@ int kgeo_rho_of_req(const kgeo_ray_params* curves, const double* req, const int3
   if (!curves || !req || !mbar || !varphi || !rho) return eq_fail(KGEO_E_ARG, "NULL array");
   if (ncurves > (int64_t(1) << 40) / nphi) return eq_fail(KGEO_E_ARG, "ncurves * nphi too large");
   const int64_t n = ncurves * nphi;
-  const unsigned nb = static_cast<unsigned>((n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(kg_rho_of_req, dim3(nb), dim3(kBlock), 0, static_cast<hipStream_t>(stream), curves, req, mbar,
+  const unsigned nb = static_cast<unsigned>((n + kLaneBlock - 1) / kLaneBlock);
+  hipLaunchKernelGGL(kg_rho_of_req, dim3(nb), dim3(kLaneBlock), 0, static_cast<hipStream_t>(stream), curves, req, mbar,
                      ncurves, varphi, nphi, rho, status);
   return eq_launch("kg_rho_of_req");
 }
