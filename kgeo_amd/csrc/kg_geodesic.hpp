@@ -486,9 +486,10 @@ KG_INLINE void theta_out(const ThetaConsts<T>& tc, const T& snarg, const ThState
   if (kind == THK_ORD) {
     Gt = v[TH_C] * st.EmF - v[TH_sGt];
   } else if (kind == THK_ORD_K0) {
-    // k == 0: m' = 0, am(u|0) = u (kerr_raytracing_ana.py:290-291)
+    // k == 0: m' = 0, am(u|0) = u (kerr_raytracing_ana.py:290-291), so
+    // sn = sin Phi, cn = cos Phi and sin 2 Phi = 2 sn cn
     T Phi = snarg;
-    Gt = -v[TH_B] * 0.125 * (-2.0 * Phi + ksin(2.0 * Phi)) - v[TH_sGt];
+    Gt = -v[TH_B] * 0.125 * (-2.0 * Phi + 2.0 * (st.s * st.c)) - v[TH_sGt];
   } else {
     Gt = v[TH_C] * (st.F + st.EmF) - v[TH_sGt];
   }

@@ -520,7 +520,9 @@ KG_INLINE void theta_direct(const ThetaConsts<T>& tc, const JacobiLadder<T>& lad
   st.s = ts.s;
   st.c = ts.c;
   st.d = ts.d;
-  st.D = DO_PHI_T ? ts.EmF / tc.v[TH_k] : T(0.0);
+  // k = 0 (|a| below MINSPIN): G_t needs no E (theta_out), and (E - F)/k is
+  // not formed; the walked D stays finite and unused
+  st.D = (DO_PHI_T && KIND != THK_ORD_K0) ? ts.EmF / tc.v[TH_k] : T(0.0);
   st.P[0] = T(0.0);
   st.Lg[0] = T(0.0);
   if (DO_PHI_T) walk_enter(w, 0, ts.Pi, arg, ts.s, ts.c, ts.d, st);
@@ -916,7 +918,8 @@ KG_INLINE ThetaPick theta_pick(const double* __restrict__ plan, int64_t stride, 
   ThetaPick tp;
   tp.stop = plan[static_cast<int64_t>(PL_STOP) * stride + p];
   tp.kind = (eta0 && (gflags & GF_ANY_ETA_POS) == 0) ? THK_NONE : kind;
-  tp.walk = walk_ok(o, (gflags & GF_ANY_STEP_ZERO) != 0) && (tp.kind == THK_ORD || tp.kind == THK_VORT);
+  tp.walk = walk_ok(o, (gflags & GF_ANY_STEP_ZERO) != 0) &&
+            (tp.kind == THK_ORD || tp.kind == THK_ORD_K0 || tp.kind == THK_VORT);
   return tp;
 }
 
@@ -929,8 +932,12 @@ KG_INLINE uint32_t theta_head_pixel(const double* __restrict__ plan, int64_t str
   if (!tp.walk) return 0u;
   JacobiLadder<T> lad;
   jacobi_ladder(tc.v[TH_mp], lad);
-  const bool bad = (tp.kind == THK_ORD) ? theta_head<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p)
-                                        : theta_head<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p);
+  bool bad;
+  switch (tp.kind) {
+    case THK_ORD: bad = theta_head<T, THK_ORD, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p); break;
+    case THK_ORD_K0: bad = theta_head<T, THK_ORD_K0, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p); break;
+    default: bad = theta_head<T, THK_VORT, DO_PHI_T>(o, tc, lad, i, hd, tp.stop, rt, p); break;
+  }
   return loop_status(bad, lad);
 }
 
@@ -941,8 +948,12 @@ KG_INLINE uint32_t theta_far_pixel(const double* __restrict__ plan, int64_t stri
   ThetaConsts<T> tc;
   const ThetaPick tp = theta_pick(plan, stride, p, gflags, o, tc);
   if (!tp.walk) return 0u;
-  const bool bad = (tp.kind == THK_ORD) ? theta_far<T, THK_ORD, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p)
-                                        : theta_far<T, THK_VORT, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p);
+  bool bad;
+  switch (tp.kind) {
+    case THK_ORD: bad = theta_far<T, THK_ORD, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p); break;
+    case THK_ORD_K0: bad = theta_far<T, THK_ORD_K0, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p); break;
+    default: bad = theta_far<T, THK_VORT, DO_PHI_T>(o, tc, i, blk, tp.stop, rt, p); break;
+  }
   return bad ? ST_NONFINITE : 0u;
 }
 
@@ -956,9 +967,12 @@ KG_INLINE uint32_t theta_walk_pixel(const double* __restrict__ plan, int64_t str
     for (int64_t js = j0; js < j1; js += kFixupSpan) push(i, js, false);
     return 0u;
   }
-  const bool bad = (tp.kind == THK_ORD)
-                       ? theta_walk<T, THK_ORD, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push)
-                       : theta_walk<T, THK_VORT, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push);
+  bool bad;
+  switch (tp.kind) {
+    case THK_ORD: bad = theta_walk<T, THK_ORD, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push); break;
+    case THK_ORD_K0: bad = theta_walk<T, THK_ORD_K0, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push); break;
+    default: bad = theta_walk<T, THK_VORT, DO_PHI_T>(o, tc, i, j0, j1, tp.stop, rt, p, push); break;
+  }
   return bad ? ST_NONFINITE : 0u;
 }
 

@@ -97,6 +97,7 @@ CASES = [
     (0.99, 88.0, 8.0, 30, 130, 2),
     (0.5, 140.0, 6.0, 24, 50, 1),
     (5.0e-5, 45.0, 8.0, 24, 80, 1),
+    (0.0, 60.0, 8.0, 24, 120, 2),  # |a| < MINSPIN: theta walks at k = 0 (sn = sin, cn = cos)
     (-0.9, 80.0, 10.0, 40, 400, 7),
     (-0.3, 3.0, 12.0, 40, 30, 1),  # ngeo below one segment: queued ranges end at the last sample
     (0.082, 62.9, 8.0, 32, 160, 4),  # low spin: many radial blocks go through the repair pass
