@@ -21,7 +21,8 @@
 //
 // A step is taken only when it is well conditioned: no pole of the integrand
 // (1 - n sn^2 = 0, n > 1) between u and u + v or at p = 0, |z^2| <= kStepZ2
-// (where the 15-term series of R_C(1, 1 - z^2) is exact to rounding) and
+// (where at most 12 terms of the series of R_C(1, 1 - z^2) are exact to
+// rounding) and
 // every stepped value finite.  Otherwise the sample is evaluated directly
 // and the walk continues from there.  The loops also restart from a direct
 // evaluation every kAnchorEveryLong samples (theta, radial case 2) or
