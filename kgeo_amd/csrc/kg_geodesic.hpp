@@ -477,9 +477,9 @@ KG_INLINE void theta_out(const ThetaConsts<T>& tc, const T& snarg, const ThState
   const T* v = tc.v;
   const int kind = (KIND >= 0) ? KIND : tc.kind;
   if (kind == THK_VORT) {
-    th = kacos(v[TH_h] * ksqrt(v[TH_A] + v[TH_B] * (st.s * st.s)));
+    th = acos_poly(v[TH_h] * ksqrt(v[TH_A] + v[TH_B] * (st.s * st.s)));
   } else {
-    th = kacos(v[TH_A] * st.s);
+    th = acos_poly(v[TH_A] * st.s);
   }
   if (!do_phi_t) return;
   Gph = v[TH_pref] * st.Pi - v[TH_sGph];

@@ -287,12 +287,13 @@ void kgh_model_eval(const kgeo_emission_model* M, double a, const double* r, con
   }
 }
 
-// Special functions of the field models: which 0 = Li2(x) (x <= 1),
-// 1 = Lambert W0(x), 2 = BZ f(r) with f'(r) in out2.
+// Special functions: which 0 = Li2(x) (x <= 1), 1 = Lambert W0(x), 2 = BZ
+// f(r) with f'(r) in out2 (the field models), 3 = the theta outputs' arccos.
 void kgh_special(int which, const double* x, int64_t n, double* out, double* out2) {
   for (int64_t i = 0; i < n; ++i) {
     if (which == 0) out[i] = li2(x[i]);
     else if (which == 1) out[i] = lambert_w0(x[i]);
+    else if (which == 3) out[i] = acos_poly(x[i]);
     else bz_f(x[i], out[i], out2[i]);
   }
 }

@@ -86,6 +86,38 @@ KG_INLINE double frcp(double x) {
   return 1.0 / x;
 #endif
 }
+// arccos on [-1, 1] (NaN outside, as numpy's arccos), within 1 ulp, for
+// the theta outputs: with it the theta walk holds four waves per SIMD (the
+// libm call's registers kept it at three).  The classical half-angle reduction (e.g. Muller, Elementary
+// Functions, ch. 11): for |x| < 1/2, acos x = pi/2 - asin x; otherwise
+// acos |x| = 2 asin s, s = sqrt(z), z = (1 - |x|)/2 (exact), and
+// acos(-|x|) = pi - acos |x|.  asin s = s + s z P(z) on z = s^2 in [0, 1/4],
+// where P is a degree-11 Chebyshev fit of (asin sqrt z - sqrt z)/(z sqrt z)
+// (mpmath, 50 digits; fit error 2.3e-16, so 6e-17 relative in asin),
+// evaluated by Estrin's scheme.  Both ranges share one polynomial
+// evaluation (selects, no branch).
+template <class T> KG_INLINE T acos_poly(const T& x) {
+  const bool small = kfabs(val(x)) < 0.5;  // false for NaN
+  const T ax = kfabs(x);
+  const T z = small ? x * x : 0.5 * (1.0 - ax);
+  const T s = small ? ax : ksqrt(z);
+  // Estrin: pairs, then z^2, z^4, z^8
+  const T z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  const T q0 = 0.1666666666666665 + z * 0.07500000000020764;
+  const T q1 = 0.044642857103423646 + z * 0.03038194736709848;
+  const T q2 = 0.02237204763174451 + z * 0.017355259955786323;
+  const T q3 = 0.013929652902326633 + z * 0.011875494382636922;
+  const T q4 = 0.0078029494773533175 + z * 0.01603551434914882;
+  const T q5 = -0.010749050339697808 + z * 0.028169218060881414;
+  const T p = (q0 + z2 * q1) + z4 * (q2 + z2 * q3) + z8 * (q4 + z2 * q5);
+  const T r = s + s * (z * p);  // asin s
+  // pi/2 and pi as a double plus its rounding error
+  constexpr double kPio2Hi = 1.5707963267948966, kPio2Lo = 6.123233995736766e-17;
+  const bool neg = val(x) < 0.0;
+  if (small) return kPio2Hi - ((neg ? -r : r) - kPio2Lo);
+  return neg ? 2.0 * kPio2Hi - (2.0 * r - 2.0 * kPio2Lo) : 2.0 * r;
+}
+
 KG_INLINE double kmax(double a, double b) { return a > b ? a : b; }
 KG_INLINE double kmin(double a, double b) { return a < b ? a : b; }
 KG_INLINE double parity_sign(double j) { return (::fmod(j, 2.0) == 0.0) ? 1.0 : -1.0; }

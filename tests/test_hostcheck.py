@@ -36,6 +36,27 @@ def test_carlson_against_scipy(host):
     assert rel(host.rc(xc, yc2), sp.elliprc(xc, yc2)) < 2e-15
 
 
+def test_acos_poly_against_mpmath(host):
+    """The theta outputs' arccos (kg_math.hpp acos_poly: half-angle reduction
+    and a degree-11 polynomial for asin) within 1 ulp of the exact value, as
+    numpy's arccos, on both sides of the |x| = 1/2 switch, next to +-1 and 0,
+    and NaN outside [-1, 1]."""
+    import mpmath as mp
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-1, 1, 4000), 1 - 10.0 ** rng.uniform(-16, 0, 500),
+                        -1 + 10.0 ** rng.uniform(-16, 0, 500), 0.5 + rng.uniform(-1e-12, 1e-12, 200),
+                        -0.5 + rng.uniform(-1e-12, 1e-12, 200), 10.0 ** rng.uniform(-300, 0, 300),
+                        [0.0, -0.0, 1.0, -1.0, 0.5, -0.5, np.nextafter(0.5, 0), np.nextafter(-0.5, 0)]])
+    got, _ = host.special(3, x)
+    with mp.workdps(40):
+        ref = np.array([float(mp.acos(mp.mpf(v))) for v in x])
+    ulp = np.spacing(np.abs(ref))
+    assert np.max(np.abs(got - ref) / ulp) <= 1.0
+    assert got[x == 1.0][0] == 0.0 and got[x == -1.0][0] == np.pi
+    bad, _ = host.special(3, np.array([1.0 + 2e-16, -1.5, np.nan, np.inf]))
+    assert np.isnan(bad).all()
+
+
 def test_rj_principal_value_matches_transform(host):
     # the reference's DLMF 19.20.14 transform (scipy_ellip_binding.py:224-230) with z = 1
     rng = np.random.default_rng(1)
