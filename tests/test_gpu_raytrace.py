@@ -216,6 +216,7 @@ def test_walk_matches_direct_evaluation():
 
 @pytest.mark.parametrize("case", [
     (0.0, 30.0, 12.0, 40, 30),        # |a| < MINSPIN branches
+    (0.0, 75.0, 10.0, 24, 600),       # |a| < MINSPIN over three 256-sample blocks (theta walks at k = 0)
     (-0.9, 80.0, 12.0, 60, 40),       # negative spin, high inclination (case-2 region II rays)
     (0.7, 140.0, 10.0, 37, 25),       # th_o > pi/2, ragged pixel count
     (0.99, 5.0, 12.0, 50, 2),         # ngeo = 2
