@@ -454,11 +454,24 @@ int require_phi_t(const kgeo_ray_params* p) {
   return KGEO_OK;
 }
 
+// Sizes one call handles: a fixup item packs the pixel in 32 bits and the
+// sample in 31, and the queue counters are 32-bit (a pass queues at most
+// npix * ceil(ngeo / 8) items).  Larger jobs are split over pixels.
+int size_limit(int64_t npix, int64_t ngeo) {
+  if (npix > 0x7fffffffLL) return fail(KGEO_E_ARG, "npix above 2^31 - 1%s (%lld): split the pixels", "", npix);
+  if (ngeo > 0x7fffffffLL) return fail(KGEO_E_ARG, "ngeo above 2^31 - 1%s (%lld)", "", ngeo);
+  if (npix > 0 && ngeo > 2 && npix * fixup_items(ngeo) > 0xffffffffLL)
+    return fail(KGEO_E_ARG, "npix * ceil(ngeo / 8) above 2^32%s (%lld): split the pixels", "",
+                npix * fixup_items(ngeo));
+  return KGEO_OK;
+}
+
 int validate(const kgeo_ray_params* p, const double* alpha, const double* beta, void* ws, size_t wsb,
              int64_t ngeo) {
   if (!p) return fail_msg(KGEO_E_ARG, "params is NULL");
   if (require_phi_t(p)) return KGEO_E_ARG;
   if (p->npix < 0) return fail(KGEO_E_ARG, "npix < 0%s (%lld)", "", p->npix);
+  if (size_limit(p->npix, ngeo)) return KGEO_E_ARG;
   if (p->npix > 0 && (!alpha || !beta)) return fail_msg(KGEO_E_ARG, "alpha/beta is NULL");
   if (p->npix > 0 && !ws) return fail_msg(KGEO_E_WORKSPACE, "workspace is NULL");
   const size_t need = kgeo_workspace_bytes(p->npix, ngeo);
@@ -644,6 +657,7 @@ int kgeo_sample(const kgeo_ray_params* p, const double* tau_in, int64_t ntau, do
   if (!p) return fail_msg(KGEO_E_ARG, "params is NULL");
   if (require_phi_t(p)) return KGEO_E_ARG;
   if (p->npix < 0 || ntau < 0) return fail_msg(KGEO_E_ARG, "negative size");
+  if (size_limit(p->npix, tau_in ? 0 : ntau)) return KGEO_E_ARG;
   if (p->npix > 0 && !workspace) return fail_msg(KGEO_E_WORKSPACE, "workspace is NULL");
   const size_t need = kgeo_workspace_bytes(p->npix, tau_in ? 0 : ntau);
   if (workspace_bytes < need)

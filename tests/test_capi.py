@@ -95,6 +95,23 @@ def test_argument_errors(bad):
     assert lib.kgeo_last_error()
 
 
+def test_sizes_beyond_the_queue_encoding_are_refused():
+    """One call handles npix < 2^31, ngeo < 2^31 and npix * ceil(ngeo / 8)
+    fixup items within 32 bits (the queue's item and counter widths); larger
+    jobs are refused before any device work, with a message to split them."""
+    lib = _lib.load()
+    buf = (ctypes.c_char * 8192)()
+    fake = ctypes.c_void_p((ctypes.addressof(buf) + 255) // 256 * 256)
+    p = _lib.make_params(0.5, 1e4, 0.5, 1 << 31, 10, 1.0)
+    rc = lib.kgeo_raytrace_ana(ctypes.byref(p), fake, fake, None, fake, None, None, fake, None, None, None, None,
+                               None, None, fake, 1 << 40, None)
+    assert rc == -1 and b"split the pixels" in lib.kgeo_last_error()
+    p = _lib.make_params(0.5, 1e4, 0.5, 1 << 24, 1 << 12, 1.0)   # 2^24 * 2^9 = 2^33 items
+    rc = lib.kgeo_sample(ctypes.byref(p), None, 1 << 12, None, fake, fake, fake, fake, None, None, fake, 1 << 50,
+                         None)
+    assert rc == -1 and b"2^32" in lib.kgeo_last_error()
+
+
 def test_every_raytrace_entry_requires_phi_t():
     """t, phi and sigma are always sampled (the reference's raytrace_ana does,
     kerr_raytracing_ana.py:97): requests without KGEO_DO_PHI_T are refused
