@@ -512,11 +512,20 @@ constexpr int kMaxDev = 64;
 std::mutex g_side_mu;
 SideStream g_side[kMaxDev];
 
-// The device's side stream, or nullptr (the passes then run in order on the
-// caller's stream).  Call with g_side_mu held.
-SideStream* side_stream() {
+// The side stream of the current device, or nullptr (the passes then run in
+// order on the caller's stream): also when the caller's stream belongs to
+// another device than the current one, whose side stream it cannot fork to.
+// Call with g_side_mu held.
+SideStream* side_stream(hipStream_t st) {
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+  if (st) {
+    hipDevice_t sd = -1;
+    if (hipStreamGetDevice(st, &sd) != hipSuccess || static_cast<int>(sd) != dev) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
   SideStream& s = g_side[dev];
   if (!s.tried) {
     s.tried = true;
@@ -555,7 +564,7 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
   const int64_t nblk = anchor_blocks(d.ngeo), nhead = anchor_heads(d.ngeo);
   {
     std::lock_guard<std::mutex> lock(g_side_mu);
-    SideStream* ss = side_stream();
+    SideStream* ss = side_stream(st);
     const hipStream_t rs = ss ? ss->side : st;
     if (ss) {
       if (hipEventRecord(ss->fork, st) != hipSuccess || hipStreamWaitEvent(rs, ss->fork, 0) != hipSuccess)
