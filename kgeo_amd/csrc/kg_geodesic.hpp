@@ -1119,7 +1119,8 @@ template <class T> KG_INLINE T mino_total(const RayScalars<T>& c, const PixelSet
 // is evaluated without FMA contraction so that its rounding is numpy's:
 // u_+ = dth + sqrt(dth^2 + eta/a^2) cancels to ~1e-5 for near-polar rays,
 // and the reference's own rounding there is what the samples inherit.
-template <class T>
+// TAU = false skips tau_tot (left 0), for callers that need the roots only.
+template <class T, bool TAU = true>
 KG_INLINE void pixel_setup(const RayScalars<T>& c, const T& alpha, const T& beta, PixelSetup<T>& px) {
 #if defined(__clang__)
 #pragma clang fp contract(off)
@@ -1134,7 +1135,7 @@ KG_INLINE void pixel_setup(const RayScalars<T>& c, const T& alpha, const T& beta
   px.thclass = ev > 0.0 ? 1 : (ev < 0.0 ? 2 : (ev == 0.0 ? 3 : 0));
   if (c.minspin && !(ev > 0.0)) px.status |= ST_MINSPIN_VORTICAL;
   radial_roots(c, px.lam, px.eta, px);
-  px.tau_tot = mino_total(c, px);
+  px.tau_tot = TAU ? mino_total(c, px) : T(0.0);
 }
 
 // Mino-time grid value j of numpy.linspace(0, maxtaufrac*tau_tot, G)
@@ -1211,15 +1212,27 @@ template <class T> struct PixelConsts {
   T stop;
 };
 
-KG_INLINE void plan_pack(const ThetaConsts<double>& tc, const RadialConsts<double>& rc, double lam, double stop,
-                         double* plan) {
+// The plan in two parts: theta, lambda and the grid stop; the radial
+// constants (PL_R .. PL_R + R_NSLOT - 1 and PL_RCASE).
+KG_INLINE void plan_pack_theta(const ThetaConsts<double>& tc, double lam, double stop, double* plan) {
   for (int i = 0; i < TH_NSLOT; ++i) plan[PL_TH + i] = tc.v[i];
-  for (int i = 0; i < R_NSLOT; ++i) plan[PL_R + i] = rc.v[i];
   plan[PL_LAM] = lam;
   plan[PL_STOP] = stop;
   plan[PL_THK] = static_cast<double>(tc.kind);
   plan[PL_ETA0] = static_cast<double>(tc.eta_zero);
+}
+
+KG_INLINE void plan_pack_radial(const RadialConsts<double>& rc, double* plan) {
+  for (int i = 0; i < R_NSLOT; ++i) plan[PL_R + i] = rc.v[i];
   plan[PL_RCASE] = static_cast<double>(rc.rcase);
+}
+
+KG_INLINE bool plan_slot_radial(int s) { return (s >= PL_R && s < PL_R + R_NSLOT) || s == PL_RCASE; }
+
+KG_INLINE void plan_pack(const ThetaConsts<double>& tc, const RadialConsts<double>& rc, double lam, double stop,
+                         double* plan) {
+  plan_pack_theta(tc, lam, stop, plan);
+  plan_pack_radial(rc, plan);
 }
 
 KG_INLINE void plan_unpack(const RayScalars<double>& c, const double* plan, PixelConsts<double>& pc) {
@@ -1264,6 +1277,27 @@ KG_INLINE PixelSetup<double> plan_pixel(const RayScalars<double>& c, double alph
   radial_consts(c, px, do_phi_t, rc);
   plan_pack(tc, rc, px.lam, c.maxtaufrac * px.tau_tot, plan);
   return px;
+}
+
+// The plan without its radial constants (plan_pack_theta's slots).
+KG_INLINE PixelSetup<double> plan_pixel_theta(const RayScalars<double>& c, double alpha, double beta, bool do_phi_t,
+                                              double* plan) {
+  PixelSetup<double> px;
+  pixel_setup(c, alpha, beta, px);
+  ThetaConsts<double> tc;
+  theta_consts(c, px, do_phi_t, tc);
+  plan_pack_theta(tc, px.lam, c.maxtaufrac * px.tau_tot, plan);
+  return px;
+}
+
+// The radial constants alone (plan_pack_radial's slots), from the same
+// per-pixel set-up, so that they equal plan_pixel's bit for bit.
+KG_INLINE void plan_pixel_radial(const RayScalars<double>& c, double alpha, double beta, bool do_phi_t, double* plan) {
+  PixelSetup<double> px;
+  pixel_setup<double, false>(c, alpha, beta, px);
+  RadialConsts<double> rc;
+  radial_consts(c, px, do_phi_t, rc);
+  plan_pack_radial(rc, plan);
 }
 
 // Geodesics.nmax_eq (kerr_raytracing_utils.py:63-67, 605-684) at the last

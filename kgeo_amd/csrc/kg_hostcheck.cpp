@@ -130,9 +130,12 @@ void kgh_raytrace_passes(const double* sc, const double* alpha, const double* be
   std::vector<double> tt(npix);
   std::vector<uint32_t> st(npix, 0u);
   uint32_t gflags = 0;
+  // the plan in the two halves the plan kernels write (kg_setup_theta,
+  // kg_setup_radial)
   for (int64_t i = 0; i < npix; ++i) {
-    SetupOut so = setup_pixel(c, ngeo, do_phi_t != 0, alpha[i], beta[i], plan.data(), npix, i, rcl.data(),
-                              tcl.data(), nmax_eq, tt.data(), st.data(), i);
+    SetupOut so = setup_pixel<true>(c, ngeo, do_phi_t != 0, alpha[i], beta[i], plan.data(), npix, i, rcl.data(),
+                                    tcl.data(), nmax_eq, tt.data(), st.data(), i);
+    setup_radial_pixel(c, do_phi_t != 0, alpha[i], beta[i], plan.data(), npix, i);
     if (so.eta_pos) gflags |= GF_ANY_ETA_POS;
     if (so.step_zero) gflags |= GF_ANY_STEP_ZERO;
   }
@@ -285,6 +288,32 @@ void kgh_model_eval(const kgeo_emission_model* M, double a, const double* r, con
     const Angle an = th ? angle_of(th[i]) : equator();
     model_point(*M, a, r[i], an, i, n, g, sinthb, bmag, u, tet, B, omega, bsq, j);
   }
+}
+
+// The plan of npix pixels written whole (kg_setup) and in its two halves
+// (kg_setup_theta, kg_setup_radial): the number of slots that differ in any
+// bit, and of label, tau_tot, nmax_eq or status values that differ.
+int64_t kgh_plan_split_mismatches(const double* sc, const double* alpha, const double* beta, int64_t npix,
+                                  int64_t ngeo) {
+  RayScalars<double> c = scalars_from(sc);
+  std::vector<double> p1(static_cast<size_t>(npix) * PL_N, -1.0), p2(static_cast<size_t>(npix) * PL_N, -2.0);
+  std::vector<int8_t> r1(npix), r2(npix), t1(npix), t2(npix);
+  std::vector<double> u1(npix), u2(npix);
+  std::vector<int32_t> n1(npix), n2(npix);
+  std::vector<uint32_t> s1(npix, 0u), s2(npix, 0u);
+  int64_t bad = 0;
+  for (int64_t i = 0; i < npix; ++i) {
+    SetupOut a = setup_pixel(c, ngeo, true, alpha[i], beta[i], p1.data(), npix, i, r1.data(), t1.data(), n1.data(),
+                             u1.data(), s1.data(), i);
+    SetupOut b = setup_pixel<true>(c, ngeo, true, alpha[i], beta[i], p2.data(), npix, i, r2.data(), t2.data(),
+                                   n2.data(), u2.data(), s2.data(), i);
+    setup_radial_pixel(c, true, alpha[i], beta[i], p2.data(), npix, i);
+    bad += (a.eta_pos != b.eta_pos) + (a.step_zero != b.step_zero);
+    bad += (r1[i] != r2[i]) + (t1[i] != t2[i]) + (n1[i] != n2[i]) + (s1[i] != s2[i]);
+    bad += std::memcmp(&u1[i], &u2[i], sizeof(double)) != 0;
+  }
+  for (size_t k = 0; k < p1.size(); ++k) bad += std::memcmp(&p1[k], &p2[k], sizeof(double)) != 0;
+  return bad;
 }
 
 // Special functions: which 0 = Li2(x) (x <= 1), 1 = Lambert W0(x), 2 = BZ

@@ -1148,14 +1148,18 @@ struct SetupOut {
   bool step_zero;
 };
 
+// THETA_ONLY: the plan without its radial constants, which
+// setup_radial_pixel then writes (the plan kernels' split).
+template <bool THETA_ONLY = false>
 KG_INLINE SetupOut setup_pixel(const RayScalars<double>& c, int64_t ngeo, bool do_phi_t, double al, double be,
                                        double* __restrict__ plan, int64_t stride, int64_t p, int8_t* rclass,
                                        int8_t* thclass, int32_t* nmax_eq, double* tau_tot, uint32_t* status,
                                        int64_t i) {
   double pl[PL_N];
-  PixelSetup<double> px = plan_pixel(c, al, be, do_phi_t, pl);
+  PixelSetup<double> px = THETA_ONLY ? plan_pixel_theta(c, al, be, do_phi_t, pl) : plan_pixel(c, al, be, do_phi_t, pl);
 #pragma unroll
-  for (int s = 0; s < PL_N; ++s) plan[static_cast<int64_t>(s) * stride + p] = pl[s];
+  for (int s = 0; s < PL_N; ++s)
+    if (!THETA_ONLY || !plan_slot_radial(s)) plan[static_cast<int64_t>(s) * stride + p] = pl[s];
   SetupOut so;
   so.eta_pos = px.eta > 0.0;
   const double stop = pl[PL_STOP];
@@ -1170,6 +1174,17 @@ KG_INLINE SetupOut setup_pixel(const RayScalars<double>& c, int64_t ngeo, bool d
   }
   if (status) status[i] = px.status;
   return so;
+}
+
+// The radial constants of pixel i's plan column p (the other half of
+// setup_pixel<true>).
+KG_INLINE void setup_radial_pixel(const RayScalars<double>& c, bool do_phi_t, double al, double be,
+                                  double* __restrict__ plan, int64_t stride, int64_t p) {
+  double pl[PL_N];
+  plan_pixel_radial(c, al, be, do_phi_t, pl);
+#pragma unroll
+  for (int s = 0; s < PL_N; ++s)
+    if (plan_slot_radial(s)) plan[static_cast<int64_t>(s) * stride + p] = pl[s];
 }
 
 #if defined(__HIPCC__)

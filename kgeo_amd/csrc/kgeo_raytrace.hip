@@ -130,6 +130,29 @@ __global__ __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(2)))
   wave_or(gflags, so.step_zero, GF_ANY_STEP_ZERO);
 }
 
+// The plan kernel in two halves: kg_setup_theta (labels, tau_tot, flags and
+// every plan slot but the radial constants) on the caller's stream, and
+// kg_setup_radial (the radial constants, about two thirds of kg_setup's
+// time) on the side stream, where the radial passes that read them run.
+__global__ __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(2))) void kg_setup_theta(
+    DevParams p, const double* __restrict__ alpha, const double* __restrict__ beta, double* __restrict__ plan,
+    uint32_t* __restrict__ gflags, int8_t* __restrict__ rclass, int8_t* __restrict__ thclass,
+    int32_t* __restrict__ nmax_eq, double* __restrict__ tau_tot, uint32_t* __restrict__ status) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
+  SetupOut so{false, false};
+  if (i < p.npix)
+    so = setup_pixel<true>(p.c, p.ngeo, p.do_phi_t != 0, alpha[i], beta[i], plan, p.npix, i, rclass, thclass,
+                           nmax_eq, tau_tot, status, i);
+  wave_or(gflags, so.eta_pos, GF_ANY_ETA_POS);
+  wave_or(gflags, so.step_zero, GF_ANY_STEP_ZERO);
+}
+
+__global__ __launch_bounds__(kPassBlock) __attribute__((amdgpu_waves_per_eu(2))) void kg_setup_radial(
+    DevParams p, const double* __restrict__ alpha, const double* __restrict__ beta, double* __restrict__ plan) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
+  if (i < p.npix) setup_radial_pixel(p.c, p.do_phi_t != 0, alpha[i], beta[i], plan, p.npix, i);
+}
+
 // ---------------------------------------------------------------------------
 // Direct sample kernels (caller-given Mino times, ngeo <= 2): grid (pixel
 // blocks, sample chunks).  kg_theta writes theta and parks G_phi, G_t in ph,
@@ -481,20 +504,6 @@ int validate(const kgeo_ray_params* p, const double* alpha, const double* beta, 
   return KGEO_OK;
 }
 
-int run_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, int8_t* rclass, int8_t* thclass,
-             int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* ws, hipStream_t st) {
-  DevParams d = make_dev(p, p->ngeo);
-  if (d.npix == 0) return KGEO_OK;
-  uint32_t* gflags = static_cast<uint32_t*>(ws);
-  double* plan = reinterpret_cast<double*>(static_cast<char*>(ws) + kFlagBytes);
-  hipLaunchKernelGGL(kg_clear_flags, dim3(1), dim3(64), 0, st, gflags);
-  int rc = check_launch("kg_clear_flags");
-  if (rc) return rc;
-  const unsigned nb = static_cast<unsigned>((d.npix + kPassBlock - 1) / kPassBlock);
-  hipLaunchKernelGGL(kg_setup, dim3(nb), dim3(kPassBlock), 0, st, d, alpha, beta, plan, gflags, rclass, thclass, nmax_eq,
-                     tau_tot, status);
-  return check_launch("kg_setup");
-}
 
 // A second stream per device for the radial anchor passes, which depend
 // only on the plan and so run beside the theta passes (two latency-bound
@@ -506,6 +515,7 @@ int run_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, 
 struct SideStream {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t planned = nullptr;  // the side stream's radial plan constants are written
   bool tried = false;
 };
 constexpr int kMaxDev = 64;
@@ -531,12 +541,45 @@ SideStream* side_stream(hipStream_t st) {
     s.tried = true;
     if (hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.planned, hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
       s.side = nullptr;
     }
   }
   return s.side ? &s : nullptr;
+}
+
+int run_plan(const kgeo_ray_params* p, const double* alpha, const double* beta, int8_t* rclass, int8_t* thclass,
+             int32_t* nmax_eq, double* tau_tot, uint32_t* status, void* ws, hipStream_t st) {
+  DevParams d = make_dev(p, p->ngeo);
+  if (d.npix == 0) return KGEO_OK;
+  uint32_t* gflags = static_cast<uint32_t*>(ws);
+  double* plan = reinterpret_cast<double*>(static_cast<char*>(ws) + kFlagBytes);
+  hipLaunchKernelGGL(kg_clear_flags, dim3(1), dim3(64), 0, st, gflags);
+  int rc = check_launch("kg_clear_flags");
+  if (rc) return rc;
+  const unsigned nb = static_cast<unsigned>((d.npix + kPassBlock - 1) / kPassBlock);
+  std::lock_guard<std::mutex> lock(g_side_mu);
+  SideStream* ss = side_stream(st);
+  if (!ss) {
+    hipLaunchKernelGGL(kg_setup, dim3(nb), dim3(kPassBlock), 0, st, d, alpha, beta, plan, gflags, rclass, thclass,
+                       nmax_eq, tau_tot, status);
+    return check_launch("kg_setup");
+  }
+  // the radial constants on the side stream, after everything enqueued so
+  // far on the caller's (an earlier sample pass may still read the plan);
+  // kgeo_sample's radial passes follow them there, and its direct path
+  // waits for `planned`
+  if (hipEventRecord(ss->fork, st) != hipSuccess || hipStreamWaitEvent(ss->side, ss->fork, 0) != hipSuccess)
+    return fail_msg(KGEO_E_HIP, "fork to the side stream failed");
+  hipLaunchKernelGGL(kg_setup_theta, dim3(nb), dim3(kPassBlock), 0, st, d, alpha, beta, plan, gflags, rclass, thclass,
+                     nmax_eq, tau_tot, status);
+  if ((rc = check_launch("kg_setup_theta"))) return rc;
+  hipLaunchKernelGGL(kg_setup_radial, dim3(nb), dim3(kPassBlock), 0, ss->side, d, alpha, beta, plan);
+  if ((rc = check_launch("kg_setup_radial"))) return rc;
+  if (hipEventRecord(ss->planned, ss->side) != hipSuccess) return fail_msg(KGEO_E_HIP, "plan event failed");
+  return KGEO_OK;
 }
 
 template <bool DO_PHI_T>
@@ -612,6 +655,13 @@ int run_sample(const kgeo_ray_params* p, int64_t ngeo, const double* tau_in, dou
   // the linspace grid walks (heads, far, walk, fixup passes); caller-given
   // Mino times and grids of at most two samples are evaluated directly
   if (!tau_in && ngeo > 2) return run_walk<true>(d, a, ws, grid, st);
+  {
+    // the plan's radial constants come from the side stream (run_plan)
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    SideStream* ss = side_stream(st);
+    if (ss && hipStreamWaitEvent(st, ss->planned, 0) != hipSuccess)
+      return fail_msg(KGEO_E_HIP, "wait for the plan failed");
+  }
   hipLaunchKernelGGL(kg_theta<true>, grid, dim3(kBlock), 0, st, d, a);
   int rc = check_launch("kg_theta");
   if (rc) return rc;

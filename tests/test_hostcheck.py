@@ -36,6 +36,23 @@ def test_carlson_against_scipy(host):
     assert rel(host.rc(xc, yc2), sp.elliprc(xc, yc2)) < 2e-15
 
 
+@pytest.mark.parametrize("a,th_deg", [(0.94, 17.0), (0.0, 60.0), (5e-5, 45.0), (-0.9, 80.0), (0.7, 140.0)])
+def test_plan_halves_equal_whole_plan(host, a, th_deg):
+    """The plan kernels' split (kg_setup_theta, then kg_setup_radial on the
+    side stream) writes every plan slot, label, tau_tot, nmax_eq and status
+    bit for bit as the whole-plan kernel kg_setup does."""
+    import ctypes
+    A, B = grid(9.0, 40)
+    A = np.concatenate([A, [0.0, 1e-9, 3.0]])
+    B = np.concatenate([B, [5.0, 0.0, 0.0]])
+    sc = host.scalars(a, 1.0e4, np.deg2rad(th_deg))
+    fn = host.lib.kgh_plan_split_mismatches
+    fn.restype = ctypes.c_int64
+    bad = fn(host._p(sc), host._p(np.ascontiguousarray(A)), host._p(np.ascontiguousarray(B)),
+             ctypes.c_int64(len(A)), ctypes.c_int64(500))
+    assert bad == 0
+
+
 def test_acos_poly_against_mpmath(host):
     """The theta outputs' arccos (kg_math.hpp acos_poly: half-angle reduction
     and a degree-11 polynomial for asin) within 1 ulp of the exact value, as
