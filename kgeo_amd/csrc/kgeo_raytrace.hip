@@ -2,14 +2,17 @@
 // Kerr raytracer.
 //
 // Pipeline of one kgeo_raytrace_ana call, on the caller's stream except the
-// radial heads, far and repair passes, which run on a per-device side stream
-// beside the theta passes and are joined before the radial walk:
+// plan's radial constants and the radial heads, far and repair passes, which
+// run on a per-device side stream beside the theta work and are joined
+// before the radial walk:
 //   kg_clear_flags   one wave: zero the flag page of the workspace
 //   kg_setup         one lane per pixel: lambda, eta, u_+-, roots, labels,
 //                    tau_tot, nmax_eq and the per-pixel constants of the
 //                    theta and radial integrals -> slot-major plan [PL_N][npix];
 //                    wave-ballot OR of the image-wide flags the reference's
-//                    numpy code decides globally (any eta > 0, any zero step)
+//                    numpy code decides globally (any eta > 0, any zero step);
+//                    with a side stream, as kg_setup_theta (all but the
+//                    radial constants) and kg_setup_radial (side stream)
 // then, on the linspace grid, per coordinate (theta first: its G_phi, G_t
 // are parked in the phi and t outputs; the radial stores combine them):
 //   kg_*_heads       one lane per (pixel, head): direct evaluations (Landen
