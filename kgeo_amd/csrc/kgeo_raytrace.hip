@@ -229,12 +229,20 @@ __device__ inline void unpack_item(unsigned long long e, int64_t& i, int64_t& j,
 
 // grid (pixel blocks, heads): the direct evaluations of the anchor pass
 template <bool DO_PHI_T>
-__global__ __launch_bounds__(kPassBlock) void kg_theta_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead) {
+// last = false: every head but the last sample's (head 2), which the theta
+// far and walk passes do not read; last = true: head 2 alone (grid y = 1),
+// which the side stream takes (it emits the last sample for the radial walk)
+__global__ __launch_bounds__(kPassBlock) void kg_theta_heads(DevParams p, SampleArgs a, Records rt, int64_t nhead,
+                                                             int last) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kPassBlock + threadIdx.x;
   if (i >= p.npix) return;
   uint32_t st = 0u;
-  for (int64_t hd = blockIdx.y; hd < nhead; hd += gridDim.y)
-    st |= theta_head_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, hd, rt);
+  if (last) {
+    st = theta_head_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, 2, rt);
+  } else {
+    for (int64_t y = blockIdx.y; y < nhead - 1; y += gridDim.y)
+      st |= theta_head_pixel<DO_PHI_T>(a.plan, p.npix, i, a.gflags[0], a.out, i, y < 2 ? y : y + 1, rt);
+  }
   if (a.status && st) atomicOr(&a.status[i], st);
 }
 
@@ -624,9 +632,13 @@ int run_walk(const DevParams& d, const SampleArgs& a, void* ws, dim3 grid, hipSt
     if ((rc = check_launch("kg_radial_far"))) return rc;
     hipLaunchKernelGGL(kg_radial_repair<DO_PHI_T>, rgrid, dim3(kPassBlock), 0, rs, d, a, rr, qrep);
     if ((rc = check_launch("kg_radial_repair"))) return rc;
+    // theta's last-sample head: read by the radial walk only
+    hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, 1), dim3(kPassBlock), 0, rs, d, a, rth, nhead, 1);
+    if ((rc = check_launch("kg_theta_heads (last sample)"))) return rc;
     if (ss && hipEventRecord(ss->join, rs) != hipSuccess) return fail_msg(KGEO_E_HIP, "join event failed");
     // theta passes on the caller's stream meanwhile
-    hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead)), dim3(kPassBlock), 0, st, d, a, rth, nhead);
+    hipLaunchKernelGGL(kg_theta_heads<DO_PHI_T>, dim3(nb, recs(nhead - 1)), dim3(kPassBlock), 0, st, d, a, rth, nhead,
+                       0);
     if ((rc = check_launch("kg_theta_heads"))) return rc;
     hipLaunchKernelGGL(kg_theta_far<DO_PHI_T>, dim3(nb, recs(nblk)), dim3(kPassBlock), 0, st, d, a, rth, nblk);
     if ((rc = check_launch("kg_theta_far"))) return rc;
