@@ -60,6 +60,18 @@ KG_INLINE double kfloor(double x) { return ::floor(x); }
 KG_INLINE double krint(double x) { return ::rint(x); }
 KG_INLINE bool kfinite(double x) { return kfabs(x) < INFINITY; }  // false for NaN
 
+// Waits until the wave's vector memory operations are complete (s_waitcnt
+// vmcnt(0); nothing on the host).  A loop that reads registers loaded just
+// before it calls this between the loads and the loop: the compiler's wait
+// insertion otherwise keeps a wait for those loads inside the loop body, and
+// as the counter completes in order, that wait then also holds each iteration
+// until the previous iteration's stores have completed.
+KG_INLINE void wait_vector_memory() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) expcnt(7) lgkmcnt(15): vmcnt only
+#endif
+}
+
 // True on every active lane of the wave when p holds on any of them (the
 // host build evaluates one lane at a time: p itself).
 KG_INLINE bool wave_any(bool p) {

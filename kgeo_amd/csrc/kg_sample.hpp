@@ -604,6 +604,7 @@ KG_INLINE bool theta_far(const SampleOut& o, const ThetaConsts<T>& tc, int64_t i
   record_load<T, 1>(rt, 1, kThFields, col, w.b, static_cast<T*>(nullptr));
   WalkState<T, 1> st;
   record_load<T, 1>(rt, segment_record(jb, L), kThFields, col, st, static_cast<T*>(nullptr));
+  wait_vector_memory();
   bool bad = false;
   bool lost = false;
   for (int64_t js = jb + L; js < je; js += L) {
@@ -641,6 +642,7 @@ KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t 
   WalkStep<T, 1> w;
   theta_init(tc, w);
   record_load<T, 1>(rt, 0, kThFields, col, w.b, static_cast<T*>(nullptr));
+  wait_vector_memory();
   const int64_t last = o.ngeo - 1;
   WalkState<T, 1> st;
   bool bad = false;
@@ -649,6 +651,7 @@ KG_INLINE bool theta_walk(const SampleOut& o, const ThetaConsts<T>& tc, int64_t 
     if (j == last) continue;  // written by the anchor pass
     if (j % L == 0) {
       record_load<T, 1>(rt, segment_record(j, L), kThFields, col, st, static_cast<T*>(nullptr));
+      wait_vector_memory();  // a queued segment would leave them pending into later steps
       queued = !kfinite(val(st.s));  // unreachable: the fixup pass takes the segment
       if (queued) push_range(push, i, j, j + L, last);
       continue;
@@ -786,6 +789,7 @@ KG_INLINE void radial_far(const SampleOut& o, const RadialConsts<T>& rc, int64_t
   WalkState<T, 3> st;
   T lb[3] = {T(0.0), T(0.0), T(0.0)};
   record_load<T, 3>(rt, segment_record(jb, kAnchorEvery), kRFields, col, st, kBlocks ? lb : static_cast<T*>(nullptr));
+  wait_vector_memory();
   bool lost = false;
   for (int64_t js = jb + L; js < je; js += L) {
     WalkState<T, 3> nx;
@@ -842,6 +846,7 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
   WalkStep<T, 3> w;
   radial_init<T, CASE>(rc, w);
   record_load<T, 3>(rt, 0, kRFields, col, w.b, static_cast<T*>(nullptr));
+  wait_vector_memory();
   const int64_t last = o.ngeo - 1;
   WalkState<T, 3> st;
   T lb[3] = {T(0.0), T(0.0), T(0.0)};
@@ -857,10 +862,10 @@ KG_INLINE bool radial_walk(const SampleOut& o, const RadialConsts<T>& rc, const 
       // segment to the fixup pass, which evaluates it directly.
       record_load<T, 3>(rt, segment_record(j, kAnchorEvery), kRFields, col, st,
                         kBlocks ? lb : static_cast<T*>(nullptr));
-      bool ok = kfinite(val(st.s)) && kfinite(val(st.c)) && kfinite(val(st.d)) && kfinite(val(st.D));
+      bool ok = kfinite(val(st.s)) & kfinite(val(st.c)) & kfinite(val(st.d)) & kfinite(val(st.D));
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        ok = ok && kfinite(val(st.P[q])) && kfinite(val(st.Lg[q])) && (!kBlocks || kfinite(val(lb[q])));
+        ok = ok & kfinite(val(st.P[q])) & kfinite(val(st.Lg[q])) & (!kBlocks || kfinite(val(lb[q])));
       queued = !ok;
       if (queued) {
         push_range(push, i, j, j + L, last);
