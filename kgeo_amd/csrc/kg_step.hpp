@@ -66,17 +66,17 @@ template <int NT, class T> KG_INLINE T rc_one_terms(const T& z2) {
 }
 
 // Terms the series needs for |z2| <= amax, to below rounding:
-// 12 for kStepZ2 (0.04^12 / 25 < 1e-18), 6 for 2e-3, 4 for 1e-4
-// (1e-16 / 9).  ADAPT: one count for the whole wave (a scalar branch; the
+// 12 for kStepZ2 (0.04^12 / 25 < 1e-18), 6 for 2e-3 (5e-18).  ADAPT: one
+// count for the whole wave (one vote and a scalar branch per series; the
 // walk kernels, where a wave's lanes are 64 neighbouring pixels at one
 // sample, so the choice does not depend on the launch shape); otherwise 12.
+// A third, four-term rung (|z2| <= 1e-4) saves two multiply-adds per series
+// but costs a second vote and more taken branches than it saves.
 template <bool ADAPT> KG_INLINE int rc_terms(double amax) {
-  if (!ADAPT || wave_any(!(amax <= 2.0e-3))) return 12;
-  return wave_any(amax > 1.0e-4) ? 6 : 4;
+  return (!ADAPT || wave_any(!(amax <= 2.0e-3))) ? 12 : 6;
 }
 
 template <class T> KG_INLINE T rc_one_series(const T& z2, int nt = 12) {
-  if (nt == 4) return rc_one_terms<4>(z2);
   if (nt == 6) return rc_one_terms<6>(z2);
   return rc_one_terms<12>(z2);
 }
