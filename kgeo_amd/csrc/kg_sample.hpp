@@ -998,14 +998,6 @@ KG_INLINE int64_t fixup_end(int64_t j, int64_t ngeo, bool walked) {
   return je < lim ? je : lim;
 }
 
-// Queued theta samples j .. fixup_end: the first by direct evaluation, the
-// rest walked from it (the batched survey's fixup pass, one lane per range).
-template <bool DO_PHI_T, class T = double>
-KG_INLINE uint32_t theta_fixup_pixel(const double* __restrict__ plan, int64_t stride, int64_t p, uint32_t gflags,
-                                     const SampleOut& o, int64_t i, int64_t j, bool walked) {
-  return theta_pixel<DO_PHI_T, T>(plan, stride, p, gflags, o, i, j, fixup_end(j, o.ngeo, walked));
-}
-
 // Queued theta sample j + s (0 <= s < kFixupSpan) of the range j .. fixup_end,
 // by direct evaluation, if it lies in the range: the fixup pass with one
 // sample per lane, whose lanes are independent (the pass is latency-bound:
@@ -1133,14 +1125,6 @@ KG_INLINE uint32_t radial_walk_pixel(const RayScalars<T>& c, const double* __res
     default: bad = radial_walk_case<T, 2, DO_PHI_T>(c, plan, stride, p, o, i, j0, j1, rt, push); break;
   }
   return bad ? ST_NONFINITE : 0u;
-}
-
-// Queued radial samples j .. fixup_end (as theta_fixup_pixel).
-template <bool DO_PHI_T, class T = double>
-KG_INLINE uint32_t radial_fixup_pixel(const RayScalars<T>& c, const double* __restrict__ plan, int64_t stride,
-                                      int64_t p, uint32_t gflags, const SampleOut& o, int64_t i, int64_t j,
-                                      bool walked) {
-  return radial_pixel<DO_PHI_T, T>(c, plan, stride, p, gflags, o, i, j, fixup_end(j, o.ngeo, walked));
 }
 
 // Queued radial sample j + s of the range j .. fixup_end (as theta_fixup_sample).
