@@ -221,13 +221,6 @@ KG_INLINE bool radial_block_step(const RadialConsts<T>& rc, const WalkState<T, 3
   return ok;
 }
 
-// Integrals of a radial case that can take the n > 1 transform.  Case 2:
-// only I_1's n1 = (r4 - r1)/(r3 - r1) > 1; n_+- - 1 has the sign of
-// (r_+- - r1)(r3 - r4) < 0, so n_+- < 1 (and no rounding reaches the
-// transform's n >= 1 + kTransformMinPt^2).  The mask changes no transform
-// choice and drops two of the three log walks from each case-2 step.
-template <int CASE> constexpr unsigned kRadialTransformable = (CASE == 2) ? 1u : 7u;
-
 // Radial samples j0..j1-1 on the linspace grid, walking X_j = U0 - Xs j h.
 template <class T, int CASE, bool DO_PHI_T>
 KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, const JacobiLadder<T>& lad,
@@ -241,7 +234,7 @@ KG_INLINE bool radial_loop_walk(const SampleOut& o, const RadialConsts<T>& rc, c
     radial_state<T, CASE>(rc, lad, dX, T(INFINITY), DO_PHI_T, sb);  // plain Jacobi at dX
     T n[3], pc[3];
     radial_pi_params<T, CASE>(rc, n, pc);
-    walk_init<T, 3, kRadialTransformable<CASE>>(w, m, n);
+    walk_init(w, m, n);
     w.b.s = sb.s;
     w.b.c = sb.c;
     w.b.d = sb.d;
@@ -685,7 +678,7 @@ template <class T, int CASE>
 KG_INLINE void radial_init(const RadialConsts<T>& rc, WalkStep<T, 3>& w) {
   T n[3], pc[3];
   radial_pi_params<T, CASE>(rc, n, pc);
-  walk_init<T, 3, kRadialTransformable<CASE>>(w, rc.v[R_k], n);
+  walk_init(w, rc.v[R_k], n);
 }
 
 // Radial walk state (and block logs) from a direct evaluation at argument X

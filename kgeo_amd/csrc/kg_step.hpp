@@ -120,15 +120,12 @@ template <class T, int NP> struct WalkStep {
 // else n is walked as it is.
 constexpr double kTransformMinPt = 0.05;
 
-// TRM: the slots that may take the transform (a compile-time mask, so that
-// the others' log walks compile away).
-template <class T, int NP, unsigned TRM = ~0u>
+template <class T, int NP>
 KG_INLINE void walk_init(WalkStep<T, NP>& w, const T& m, const T (&n)[NP]) {
   w.m = m;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
-    const bool cand = ((TRM >> i) & 1u) && (val(n[i]) > 1.0) && kfinite(val(n[i])) && (val(m) >= 0.0) &&
-                      (val(m) < 1.0);
+    const bool cand = (val(n[i]) > 1.0) && kfinite(val(n[i])) && (val(m) >= 0.0) && (val(m) < 1.0);
     const T pt = cand ? ksqrt((n[i] - 1.0) * (1.0 - m / n[i])) : T(0.0);
     w.tr[i] = cand && (val(pt) >= kTransformMinPt);
     const T nw = w.tr[i] ? m / n[i] : n[i];
