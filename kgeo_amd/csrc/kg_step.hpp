@@ -101,6 +101,9 @@ template <class T, int NP> struct WalkStep {
   T ipt2[NP];       // 1 / (2 pt)
   T cw[NP], ncw[NP];  // sqrt(-w2) and n / sqrt(-w2) where w2 < 0, else 0
   bool tr[NP];
+  // some lane of the wave transforms an integral other than the first
+  // (walk_init; the steps then walk every slot's log, else the first's only)
+  bool tr_rest;
 };
 
 // Characteristics n > 1 (principal value) are walked through
@@ -137,6 +140,10 @@ KG_INLINE void walk_init(WalkStep<T, NP>& w, const T& m, const T (&n)[NP]) {
     w.cw[i] = circ ? ksqrt(-w.w2[i]) : T(0.0);
     w.ncw[i] = circ ? nw / w.cw[i] : T(0.0);
   }
+  bool rest = false;
+#pragma unroll
+  for (int i = 1; i < NP; ++i) rest = rest || w.tr[i];
+  w.tr_rest = wave_any(rest);
 }
 
 // Direct entry into the walk at argument u: from Pi(n; am u), set the walked
@@ -289,10 +296,29 @@ KG_INLINE bool walk_integrals(const WalkState<T, NP>& a, const WalkStep<T, NP>& 
     out.P[i] = a.P[i] + b.P[i] + corr;
     ok = ok & kfinite(val(out.P[i]));
   }
-  // the transformed integrals' log terms, one shared reciprocal
-  T yd[NP], iyd[NP];
+  // the transformed integrals' log terms, one shared reciprocal; where no
+  // lane of the wave transforms the second or third integral, the first's
+  // alone (the same values: the general form multiplies by exact ones there)
   const T num = b.s * (1.0 - w.m * (S * S) * (a.s * a.s));
   const T cdn = C * Dn * (a.c * a.d);
+  if (NP > 1 && !w.tr_rest) {
+#pragma unroll
+    for (int i = 1; i < NP; ++i) out.Lg[i] = T(0.0);
+    if (w.tr[0]) {
+      const T y = w.pt[0] * num * frcp(cdn - (w.pt[0] * w.pt[0]) * (S * a.s));
+      const T y2 = y * y;
+      const double ay = kfabs(val(y2));
+      const int nty = rc_terms<ADAPT>(ay <= kStepZ2 ? ay : 0.0);
+      T lg = a.Lg[0] + 2.0 * y * rc_one_series(y2, nty);
+      if (!(kfabs(val(y2)) <= kStepZ2)) lg = log_ratio(w.pt[0] * S, C * Dn);
+      out.Lg[0] = lg;
+      ok = ok & (kfabs(val(lg)) <= kLogGuard);  // false for NaN
+    } else {
+      out.Lg[0] = T(0.0);
+    }
+    return ok & kfinite(val(out.D));
+  }
+  T yd[NP], iyd[NP];
 #pragma unroll
   for (int i = 0; i < NP; ++i) yd[i] = w.tr[i] ? cdn - (w.pt[i] * w.pt[i]) * (S * a.s) : T(1.0);
   shared_reciprocals<T, NP>(yd, iyd);

@@ -231,6 +231,21 @@ def test_other_regimes(case):
     _check(got, ref, a, 1e4, th, A, B, f"a={a} th={th_deg} n={n} G={G}", gate_p99=True, ill_bound=1e-2)
 
 
+def test_case2_rays_transforming_every_integral():
+    """Just outside the shadow's prograde edge at a = 0.99, 100 deg, radial
+    case 2 rays take the n > 1 transform on I_+ and I_- as well as I_1, and
+    share waves with rays that take it on I_1 only: the walk then runs every
+    log term for the whole wave (kg_step.hpp, tr_rest).  Against the oracle."""
+    a, th_deg, G = 0.99, 100.0, 120
+    th = np.deg2rad(th_deg)
+    al, be = np.linspace(-1.9, -1.5, 20), np.linspace(-0.7, 0.7, 24)
+    A, B = (x.flatten() for x in np.meshgrid(al, be))
+    ref = O.raytrace_ana(a, 1e4, th, A, B, G)
+    got = _gpu(a, 1e4, th, A, B, G)
+    _check(got, ref, a, 1e4, th, A, B, f"a={a} th={th_deg} prograde shadow edge G={G}", gate_p99=True,
+           ill_bound=1e-2)
+
+
 def test_zero_spin_near_axis_pixels():
     """a = 0 with screen points on and next to the axes (impact parameter
     down to 1e-9: radial infall) among ordinary pixels.  The reference's

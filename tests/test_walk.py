@@ -108,6 +108,32 @@ CASES = [
 def test_walking_loops_match_direct_evaluation(case):
     a, thd, w, n, G, chunks = case
     A, B = grid(w, n)
+    _walk_matches_direct(a, thd, A, B, G, chunks)
+
+
+def test_case2_rays_with_all_three_integrals_transformed():
+    """Radial case 2 (four real roots, the ray outside r4) has n1 > 1 and also
+    n_+- = (r_+- - r3)(r4 - r1) / ((r_+- - r4)(r3 - r1)) > 1, as r_+- < r4.
+    Just outside the shadow's prograde edge at high inclination r4 nears r_+
+    and the transform (pt >= 0.05) takes I_+ and I_- too, so a wave walks all
+    three log terms there rather than I_1's alone.  The screen below holds
+    such rays next to rays that transform I_1 only."""
+    a, thd = 0.99, 100.0
+    al, be = np.linspace(-1.9, -1.5, 20), np.linspace(-0.7, 0.7, 24)
+    A, B = (x.flatten() for x in np.meshgrid(al, be))
+    s = Host().setup(a, 1.0e4, np.deg2rad(thd), A, B)
+    r1, r2, r3, r4 = s["roots"][:, 0::2].T
+    m = ((r3 - r2) * (r4 - r1)) / ((r3 - r1) * (r4 - r2))
+    rp = 1.0 + np.sqrt(1.0 - a * a)
+    with np.errstate(all="ignore"):
+        n = ((rp - r3) * (r4 - r1)) / ((rp - r4) * (r3 - r1))
+        pt = np.sqrt((n - 1.0) * (1.0 - m / n))
+    hit = (s["rcase"] == 2) & (m >= 0.0) & (m < 1.0) & (n > 1.0) & (pt >= 0.05)
+    assert 10 <= int(hit.sum()) < len(A) // 2, int(hit.sum())
+    _walk_matches_direct(a, thd, A, B, 120, 2)
+
+
+def _walk_matches_direct(a, thd, A, B, G, chunks):
     A, B = np.ascontiguousarray(A), np.ascontiguousarray(B)
     th = np.deg2rad(thd)
     host = Host()
